@@ -1,0 +1,184 @@
+/*
+ * ovxhip.h -- C-ABI of the MI355X (gfx950) fast-convolution backend for
+ * OpenVSIP.
+ *
+ * Every entry point returns an ovx_status (0 = success) and never throws.
+ * The message of the last failure on the calling thread is available from
+ * ovx_last_error().  Complex data is complex<float>, interleaved (re, im),
+ * passed as void*.  Strides and sizes are in elements, as in the reference's
+ * backend interface.  Unless a plan is created with OVX_FLAG_HOST_POINTERS,
+ * data pointers are device pointers (HBM-resident), and work is enqueued on
+ * the given hipStream_t (passed as void*; NULL = the default stream) without
+ * blocking the host.  A plan is not safe for concurrent calls (as the
+ * reference's Fft/Fftm objects, which own their plans and buffers); separate
+ * plans are independent.
+ *
+ * Reference interfaces replaced (paths relative to the OpenVSIP tree):
+ *   fft_fwd / fft_inv            src/ovxx/signal/fft/backend.hpp:19-22
+ *   fftm_backend<c,c,A,D>        src/ovxx/signal/fft/backend.hpp:696-784
+ *   fft_backend<1,c,c,S>         src/ovxx/signal/fft/backend.hpp:131-204
+ *   Fftm/Fft construction        src/vsip/impl/signal/fft.hpp:75-161,
+ *                                src/ovxx/fftw/fftw.cpp:904-1002
+ *   fused fastconv (be::fc_expr) src/ovxx/dispatcher/tags.hpp:33-34,
+ *                                src/ovxx/assign_fwd.hpp:58 (slot, empty
+ *                                in the reference)
+ *   Correlation                  src/ovxx/signal/corr.hpp:611-719
+ *   Convolution                  src/ovxx/signal/conv.hpp:964-1134
+ */
+#ifndef OVXHIP_H
+#define OVXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+typedef enum {
+  OVX_OK = 0,
+  OVX_ERR_INVALID = 1,     /* bad argument (null pointer, size mismatch)   */
+  OVX_ERR_UNSUPPORTED = 2, /* valid request this backend does not handle:
+                              the dispatcher should try the next backend
+                              (rt_valid() == false / "unimplemented")     */
+  OVX_ERR_NOMEM = 3,       /* device allocation failed (std::bad_alloc)    */
+  OVX_ERR_DEVICE = 4,      /* HIP runtime error                            */
+  OVX_ERR_BUDGET = 5,      /* admission refused: over the HBM budget       */
+  OVX_ERR_TIMEOUT = 6,     /* serving: request not completed in time       */
+  OVX_ERR_STATE = 7        /* call not valid in the object's current state */
+} ovx_status;
+
+/* Transform directions: the reference's values (backend.hpp:19-22). */
+#define OVX_FFT_FWD (-2)
+#define OVX_FFT_INV (-1)
+
+/* Axis of an Fftm: OVX_ROW transforms each row, OVX_COL each column
+   (vsip::row = 0, vsip::col = 1). */
+#define OVX_ROW 0
+#define OVX_COL 1
+
+/* Plan flags. */
+#define OVX_FLAG_HOST_POINTERS 0x1u /* data pointers are host memory; the call
+                                       stages through device buffers and
+                                       returns after the result is back    */
+
+/* Correlation / convolution enums (src/vsip/impl/signal/types.hpp). */
+#define OVX_SUPPORT_FULL 0
+#define OVX_SUPPORT_SAME 1
+#define OVX_SUPPORT_MIN 2
+#define OVX_BIASED 0
+#define OVX_UNBIASED 1
+#define OVX_NONSYM 0
+#define OVX_SYM_EVEN_LEN_ODD 1
+#define OVX_SYM_EVEN_LEN_EVEN 2
+
+typedef struct ovx_plan_s *ovx_plan;
+
+/* ---- library ---------------------------------------------------------- */
+/* Select the HIP device for plans created afterwards on this thread.
+   Replaces the per-thread library initialisation of vsipl
+   (src/ovxx/library.cpp:51-106). */
+int ovx_init(int device);
+const char *ovx_last_error(void);
+const char *ovx_version(void);
+/* Largest transform length the backend accepts (rt_valid upper bound). */
+long ovx_max_length(void);
+/* 1 if `n` is a transform length this backend handles, else 0. */
+int ovx_supported_length(long n);
+
+/* ---- Fftm (multiple 1-D transforms) ----------------------------------- */
+/* Create a plan for a rows x cols Fftm along `axis` in `direction`, result
+   multiplied by `scale` inside the kernel (supports_scale() == true, so the
+   reference workspace skips its separate scale pass, workspace.hpp:474).
+   Replaces Evaluator<op::fftm,...>::exec / fftm ctor
+   (src/ovxx/fftw/fft.hpp:139-145, fftw.cpp:916-919). */
+int ovx_fftm_plan_create(ovx_plan *plan, long rows, long cols, int axis,
+                         int direction, float scale, unsigned flags);
+
+/* fftm_backend::out_of_place (backend.hpp:760-770): strides s0 (between
+   rows) and s1 (between columns); the stride along the transform axis must
+   be 1 (the layout query_layout requests, backend.hpp:713-723). */
+int ovx_fftm_out_of_place(ovx_plan plan, const void *in, long in_s0, long in_s1,
+                          void *out, long out_s0, long out_s1, long rows,
+                          long cols, void *stream);
+
+/* fftm_backend::in_place (backend.hpp:735-745). */
+int ovx_fftm_in_place(ovx_plan plan, void *inout, long s0, long s1, long rows,
+                      long cols, void *stream);
+
+/* ---- Fft (one 1-D transform) ------------------------------------------- */
+/* vsip::Fft<const_Vector,C,C,dir> construction (fft.hpp:75-103). */
+int ovx_fft_plan_create(ovx_plan *plan, long n, int direction, float scale,
+                        unsigned flags);
+/* fft_backend<1,c,c,S>::out_of_place (backend.hpp:181-190); in_stride and
+   out_stride must be 1. */
+int ovx_fft_out_of_place(ovx_plan plan, const void *in, long in_stride,
+                         void *out, long out_stride, long n, void *stream);
+/* fft_backend<1,c,c,S>::in_place (backend.hpp:161-170). */
+int ovx_fft_in_place(ovx_plan plan, void *inout, long stride, long n,
+                     void *stream);
+
+/* ---- fused fast convolution ------------------------------------------- */
+/* Plan for Y = scale * inv_fftm( W (.) fwd_fftm(X) ) over rows x n, the
+   single expression  out = inv_fftm(vmmul<0>(w, for_fftm(in)))
+   (tests/fastconv.cpp:200-205, benchmarks/fastconv.cpp:797-801) and its
+   coefficient-matrix form  out = inv_fftm(C * for_fftm(in))
+   (tests/fastconv.cpp:215-245).  scale is the inverse Fftm's scale
+   (normally 1/n). */
+int ovx_fastconv_plan_create(ovx_plan *plan, long rows, long n, float scale,
+                             unsigned flags);
+/* x, y: rows x n at row strides ldx, ldy (unit column stride).  w: the
+   replica spectrum (frequency domain), one vector when ldw == 0
+   (vmmul<0>), else a coefficient matrix with row stride ldw.  In place
+   (x == y) is allowed. */
+int ovx_fastconv_exec(ovx_plan plan, const void *x, long ldx, const void *w,
+                      long ldw, void *y, long ldy, long rows, void *stream);
+
+/* ---- replica spectra kept resident on the device ---------------------- */
+/* Transform a time-domain replica of length m (m <= n), zero-padded to n,
+   once with the forward Fft (tests/fastconv.cpp:20-26 t2f) and keep the
+   spectrum on the device; when is_spectrum != 0, `replica` already holds the
+   n-point spectrum.  `replica` is a host pointer.  Returns an id. */
+int ovx_replica_create(long n, const void *replica, long m, int is_spectrum,
+                       int *id);
+int ovx_replica_destroy(int id);
+/* Device pointer to the spectrum (n complex<float>), for ovx_fastconv_exec. */
+int ovx_replica_spectrum(int id, const void **spectrum, long *n);
+
+/* ---- Correlation (FFT-based, overlap-save) ----------------------------- */
+/* vsip::Correlation<const_Vector, support, complex<float>> of a reference of
+   length m against an input of length n (corr.hpp:616-628 sizes, output
+   length from support/test/ref/corr.hpp:24-36). */
+int ovx_corr_plan_create(ovx_plan *plan, int support, long m, long n,
+                         unsigned flags);
+long ovx_corr_output_size(int support, long m, long n);
+/* Correlation::correlate (corr.hpp:663-719):
+   out[i] = sum_k ref[k] * conj(in[i + k - shift]), unbiased divisors per
+   position when bias == OVX_UNBIASED. */
+int ovx_corr_exec(ovx_plan plan, int bias, const void *ref, const void *in,
+                  void *out, void *stream);
+
+/* ---- Convolution (FFT-based, overlap-save) ----------------------------- */
+/* vsip::Convolution<const_Vector, symmetry, support, complex<float>> with
+   coefficients of length mc (expanded by symmetry, conv.hpp:144-175),
+   input length n and decimation d (output size conv.hpp:103-121). */
+int ovx_conv_plan_create(ovx_plan *plan, int symmetry, int support,
+                         const void *coeff, long mc, long n, long decimation,
+                         unsigned flags);
+long ovx_conv_output_size(int support, long m, long n, long decimation);
+int ovx_conv_exec(ovx_plan plan, const void *in, void *out, void *stream);
+
+/* ---- plans ------------------------------------------------------------- */
+int ovx_plan_destroy(ovx_plan plan);
+/* Name of the kernel path the plan runs (diagnostics, as the reference's
+   dispatch diagnostics print the chosen evaluator, dispatcher/diagnostics.hpp). */
+const char *ovx_plan_name(ovx_plan plan);
+/* Bytes of device memory the plan holds (twiddles, workspaces). */
+size_t ovx_plan_device_bytes(ovx_plan plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OVXHIP_H */

@@ -1,0 +1,9 @@
+"""openvsip_amd -- MI355X (gfx950) backend for OpenVSIP's fast-convolution path.
+
+The product is libovxhip.so (HIP kernels + C-ABI, include/ovxhip.h).  This
+package holds its sources (csrc/), the Python ctypes binding used by the
+tests and the benchmark (api.py), and seeded workload generators.
+"""
+from ._lib import OvxError, OvxInvalid, OvxLibraryMissing, OvxUnsupported, LIB_PATH  # noqa: F401
+from .api import (COL, FFT_FWD, FFT_INV, ROW, Convolution, Correlation, Fastconv, Fft,  # noqa: F401
+                  Fftm, Replica, init)

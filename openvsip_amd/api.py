@@ -1,0 +1,227 @@
+"""Python mirror of the VSIPL++ objects on the fast-convolution path.
+
+Each class wraps one C-ABI plan (include/ovxhip.h) and mirrors the reference
+object of the same name (paths relative to the OpenVSIP tree):
+
+  Fftm         vsip::Fftm            src/vsip/impl/signal/fft.hpp:134-161
+  Fft          vsip::Fft             src/vsip/impl/signal/fft.hpp:75-103
+  Fastconv     inv_fftm(vmmul<0>(w, fwd_fftm(x)))  tests/fastconv.cpp:181-213
+  Correlation  vsip::Correlation     src/vsip/impl/signal/corr.hpp:55-122
+  Convolution  vsip::Convolution     src/vsip/impl/signal/conv.hpp:55-108
+
+Data are torch complex64 tensors on the GPU (torch provides device memory and
+streams only); work is enqueued on torch's current stream.  Every call goes
+through libovxhip.so; there is no fallback path.
+"""
+import ctypes
+
+from . import _lib
+from ._lib import check, lib
+
+FFT_FWD = _lib.FFT_FWD
+FFT_INV = _lib.FFT_INV
+ROW = _lib.ROW
+COL = _lib.COL
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check_tensor(t, name):
+    torch = _torch()
+    if t.dtype != torch.complex64:
+        raise TypeError(f"{name} must be complex64 (complex<float>)")
+    if not t.is_cuda:
+        raise TypeError(f"{name} must be a GPU tensor")
+
+
+class _Plan:
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def close(self):
+        if self._h:
+            lib().ovx_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def name(self):
+        return lib().ovx_plan_name(self._h).decode()
+
+    @property
+    def device_bytes(self):
+        return lib().ovx_plan_device_bytes(self._h)
+
+
+class Fftm(_Plan):
+    """Multiple FFTs along `axis` (ROW: each row) with in-kernel scaling."""
+
+    def __init__(self, rows, cols, axis=ROW, direction=FFT_FWD, scale=1.0):
+        super().__init__()
+        self.rows, self.cols, self.axis, self.direction, self.scale = rows, cols, axis, direction, scale
+        check(lib().ovx_fftm_plan_create(ctypes.byref(self._h), rows, cols, axis, direction, scale, 0))
+
+    def __call__(self, x, out=None):
+        torch = _torch()
+        _check_tensor(x, "input")
+        if out is None:
+            out = torch.empty_like(x)
+        _check_tensor(out, "output")
+        check(lib().ovx_fftm_out_of_place(self._h, _ptr(x), x.stride(0), x.stride(1), _ptr(out),
+                                          out.stride(0), out.stride(1), self.rows, self.cols, _stream()))
+        return out
+
+    def in_place(self, x):
+        _check_tensor(x, "inout")
+        check(lib().ovx_fftm_in_place(self._h, _ptr(x), x.stride(0), x.stride(1), self.rows, self.cols,
+                                      _stream()))
+        return x
+
+
+class Fft(_Plan):
+    def __init__(self, n, direction=FFT_FWD, scale=1.0):
+        super().__init__()
+        self.n = n
+        check(lib().ovx_fft_plan_create(ctypes.byref(self._h), n, direction, scale, 0))
+
+    def __call__(self, x, out=None):
+        torch = _torch()
+        _check_tensor(x, "input")
+        if out is None:
+            out = torch.empty_like(x)
+        check(lib().ovx_fft_out_of_place(self._h, _ptr(x), x.stride(0), _ptr(out), out.stride(0), self.n,
+                                         _stream()))
+        return out
+
+
+class Fastconv(_Plan):
+    """Y = scale * IFFT_row( W (.) FFT_row(X) ) in one fused pass."""
+
+    def __init__(self, rows, n, scale=None):
+        super().__init__()
+        self.rows, self.n = rows, n
+        self.scale = 1.0 / n if scale is None else scale
+        check(lib().ovx_fastconv_plan_create(ctypes.byref(self._h), rows, n, self.scale, 0))
+
+    def __call__(self, x, w, out=None, rows=None):
+        torch = _torch()
+        _check_tensor(x, "input")
+        _check_tensor(w, "spectrum")
+        if out is None:
+            out = torch.empty_like(x)
+        rows = x.shape[0] if rows is None else rows
+        ldw = 0 if w.dim() == 1 else w.stride(0)
+        check(lib().ovx_fastconv_exec(self._h, _ptr(x), x.stride(0), _ptr(w), ldw, _ptr(out), out.stride(0),
+                                      rows, _stream()))
+        return out
+
+    def exec_raw(self, x_ptr, ldx, w_ptr, ldw, y_ptr, ldy, rows, stream):
+        return lib().ovx_fastconv_exec(self._h, x_ptr, ldx, w_ptr, ldw, y_ptr, ldy, rows, stream)
+
+
+class Replica:
+    """A replica spectrum kept resident on the device."""
+
+    def __init__(self, n, replica, is_spectrum=False):
+        import numpy as np
+        r = np.ascontiguousarray(replica, dtype=np.complex64)
+        self.n = n
+        self._id = ctypes.c_int()
+        check(lib().ovx_replica_create(n, r.ctypes.data_as(ctypes.c_void_p), r.size, int(bool(is_spectrum)),
+                                       ctypes.byref(self._id)))
+
+    def spectrum_ptr(self):
+        p = ctypes.c_void_p()
+        n = ctypes.c_long()
+        check(lib().ovx_replica_spectrum(self._id, ctypes.byref(p), ctypes.byref(n)))
+        return p
+
+    def spectrum(self):
+        """A torch view (copy) of the resident spectrum."""
+        torch = _torch()
+        out = torch.empty(self.n, dtype=torch.complex64, device="cuda")
+        torch.cuda.synchronize()
+        import ctypes as C
+        hip = _hip()
+        hip.hipMemcpy(C.c_void_p(out.data_ptr()), self.spectrum_ptr(), C.c_size_t(self.n * 8), C.c_int(3))
+        return out
+
+    def close(self):
+        if self._id.value:
+            lib().ovx_replica_destroy(self._id)
+            self._id = ctypes.c_int(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_hip_lib = None
+
+
+def _hip():
+    global _hip_lib
+    if _hip_lib is None:
+        _hip_lib = ctypes.CDLL("libamdhip64.so")
+        _hip_lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return _hip_lib
+
+
+class Correlation(_Plan):
+    def __init__(self, support, m, n):
+        super().__init__()
+        self.support, self.m, self.n = support, m, n
+        check(lib().ovx_corr_plan_create(ctypes.byref(self._h), _lib.SUPPORT[support], m, n, 0))
+        self.output_size = lib().ovx_corr_output_size(_lib.SUPPORT[support], m, n)
+
+    def __call__(self, bias, ref, x, out=None):
+        torch = _torch()
+        _check_tensor(ref, "reference")
+        _check_tensor(x, "input")
+        if out is None:
+            out = torch.empty(self.output_size, dtype=torch.complex64, device=x.device)
+        check(lib().ovx_corr_exec(self._h, _lib.BIAS[bias], _ptr(ref), _ptr(x), _ptr(out), _stream()))
+        return out
+
+
+class Convolution(_Plan):
+    def __init__(self, coeff, symmetry, support, n, decimation=1):
+        super().__init__()
+        import numpy as np
+        c = np.ascontiguousarray(coeff, dtype=np.complex64)
+        self.support, self.n, self.decimation = support, n, decimation
+        check(lib().ovx_conv_plan_create(ctypes.byref(self._h), _lib.SYMMETRY[symmetry], _lib.SUPPORT[support],
+                                         c.ctypes.data_as(ctypes.c_void_p), c.size, n, decimation, 0))
+        m = {"nonsym": c.size, "sym_even_len_odd": 2 * c.size - 1, "sym_even_len_even": 2 * c.size}[symmetry]
+        self.output_size = lib().ovx_conv_output_size(_lib.SUPPORT[support], m, n, decimation)
+
+    def __call__(self, x, out=None):
+        torch = _torch()
+        _check_tensor(x, "input")
+        if out is None:
+            out = torch.empty(self.output_size, dtype=torch.complex64, device=x.device)
+        check(lib().ovx_conv_exec(self._h, _ptr(x), _ptr(out), _stream()))
+        return out
+
+
+def init(device=0):
+    check(lib().ovx_init(device))

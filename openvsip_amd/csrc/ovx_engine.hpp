@@ -1,0 +1,139 @@
+// ovx_engine.hpp -- register-resident Stockham FFT engine for one line of
+// N complex<float> values handled by T = N/P threads (P values per thread).
+//
+// Layout contract ("slot form"): thread t holds in v[q] the element at
+// position t + T*q of the line, q in [0, P).  Global loads and stores in slot
+// form are coalesced (consecutive threads touch consecutive elements).
+//
+// Stage s has radix R_s and Ns = R_0*...*R_{s-1}.  Butterfly j (j < N/R) reads
+// x[j + r*N/R] -- exactly slots q = m + r*(P/R) of thread t where
+// j = t + m*T -- twiddles them by W_{Ns R}^{r (j mod Ns)}, runs a radix-R DFT
+// and writes output r to position (j/Ns)*Ns*R + (j mod Ns) + r*Ns (Stockham
+// autosort: natural order in, natural order out).  The last stage
+// (Ns*R = N) writes back to the very positions it read, so its outputs stay in
+// registers in slot form: a forward transform can feed an inverse transform
+// (fast convolution) without an exchange, and the final stage stores straight
+// to global memory.  Every other stage exchanges through LDS: one write of P
+// values, a barrier, one read of P values in slot form.
+//
+// LDS rows are padded by one element every 16 (phys = pos + pos/16), which
+// keeps every 16-lane group of the exchange patterns used here on distinct
+// banks.
+#pragma once
+#include "ovx_device.hpp"
+
+namespace ovx {
+
+// ---------------------------------------------------------------- plans
+struct RadixPlan {
+  int nst;
+  int radix[16];
+  int ns[16];
+  int twoff[16]; // offset of stage s in the concatenated twiddle table
+  int twtotal;
+};
+
+constexpr bool divides(int a, int b) { return b % a == 0; }
+
+// Radix sequence for length N with P values per thread: radices must divide
+// P; larger radices first (the first stage needs no twiddles, so the stage
+// with the most twiddle multiplies goes there).
+constexpr RadixPlan make_plan(int N, int P)
+{
+  RadixPlan p{};
+  p.nst = 0;
+  int rem = N;
+  const int cand[6] = { 16, 8, 4, 5, 3, 2 };
+  while (rem > 1) {
+    int chosen = 0;
+    for (int i = 0; i < 6; ++i) {
+      int r = cand[i];
+      if (divides(r, rem) && divides(r, P)) { chosen = r; break; }
+    }
+    if (chosen == 0) { p.nst = -1; return p; }
+    p.radix[p.nst++] = chosen;
+    rem /= chosen;
+  }
+  int ns = 1, off = 0;
+  for (int s = 0; s < p.nst; ++s) {
+    p.ns[s] = ns;
+    p.twoff[s] = off;
+    if (ns > 1) off += (p.radix[s] - 1) * ns;
+    ns *= p.radix[s];
+  }
+  p.twtotal = off;
+  return p;
+}
+
+constexpr int lds_pad(int pos) { return pos + (pos >> 4); }
+constexpr int lds_row_elems(int N) { return N >= 16 ? N + N / 16 : N; }
+
+// ---------------------------------------------------------------- engine
+template <int N, int P>
+struct Engine {
+  static constexpr int T = N / P;
+  static constexpr RadixPlan plan = make_plan(N, P);
+  static constexpr int NST = plan.nst;
+  static constexpr int LROW = lds_row_elems(N);
+  static_assert(N % P == 0, "P must divide N");
+  static_assert(plan.nst > 0 || N == 1, "no radix plan for N with this P");
+
+  // One stage.  tw: concatenated forward twiddle table (FP32 values of FP64
+  // W_{Ns R}^{r k}, layout tw[twoff_s + (r-1)*Ns + k]).
+  template <int S, int DIR>
+  __device__ __forceinline__ static void stage(float2 (&v)[P], float2 *lds, int t, rsrc_t tw)
+  {
+    constexpr int R = plan.radix[S];
+    constexpr int Ns = plan.ns[S];
+    constexpr int M = P / R;
+    constexpr bool last = (S == NST - 1);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      float2 a[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r] = v[m + r * M];
+      const int j = t + m * T;
+      if constexpr (Ns > 1) {
+        const int k = j % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = bload(tw, (uint32_t)k * 8u, (uint32_t)(plan.twoff[S] + (r - 1) * Ns) * 8u);
+          a[r] = twmul<DIR>(a[r], w);
+        }
+      }
+      Dft<R, DIR>::run(a);
+      if constexpr (last) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[m + r * M] = a[r];
+      } else {
+        const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lds_pad(base + r * Ns)] = a[r];
+      }
+    }
+    if constexpr (!last) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < P; ++q) v[q] = lds[lds_pad(t + T * q)];
+      __syncthreads();
+    }
+  }
+
+  template <int S, int DIR>
+  __device__ __forceinline__ static void stages_from(float2 (&v)[P], float2 *lds, int t, rsrc_t tw)
+  {
+    if constexpr (S < NST) {
+      stage<S, DIR>(v, lds, t, tw);
+      stages_from<S + 1, DIR>(v, lds, t, tw);
+    }
+  }
+
+  // Full transform of the line held in slot form; result in slot form.
+  template <int DIR>
+  __device__ __forceinline__ static void fft(float2 (&v)[P], float2 *lds, int t, rsrc_t tw)
+  {
+    stages_from<0, DIR>(v, lds, t, tw);
+  }
+};
+
+} // namespace ovx

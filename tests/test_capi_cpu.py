@@ -1,0 +1,60 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+public header declares, and its pure host functions agree with the oracle.
+No kernel is launched here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import oracle as o
+from openvsip_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "ovxhip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ovx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    # and the binding describes each of them
+    assert set(syms) == set(_lib.SIGNATURES)
+
+
+def test_output_sizes_match_oracle():
+    lib = _lib.lib()
+    for sup, code in _lib.SUPPORT.items():
+        for m in (1, 3, 8):
+            for n in (8, 9, 65536):
+                assert lib.ovx_corr_output_size(code, m, n) == o.corr_output_size(sup, m, n)
+                for d in (1, 2, 5):
+                    assert lib.ovx_conv_output_size(code, m, n, d) == o.conv_output_size(sup, m, n, d)
+
+
+def test_supported_lengths():
+    lib = _lib.lib()
+    for n in (2, 16, 2048, 3072, 12288, 16384, 10240):
+        assert lib.ovx_supported_length(n) == 1, n
+    assert lib.ovx_max_length() >= 16384
+
+
+def test_invalid_arguments_return_status_not_crash():
+    lib = _lib.lib()
+    assert lib.ovx_fftm_plan_create(None, 4, 4, 0, -2, 1.0, 0) == _lib.OVX_ERR_INVALID
+    assert b"null" in lib.ovx_last_error()
+    h = ctypes.c_void_p()
+    assert lib.ovx_fftm_plan_create(ctypes.byref(h), 4, 4, 5, -2, 1.0, 0) == _lib.OVX_ERR_INVALID
+    assert lib.ovx_fftm_plan_create(ctypes.byref(h), 4, 4, 0, 3, 1.0, 0) == _lib.OVX_ERR_INVALID
+    assert lib.ovx_corr_plan_create(ctypes.byref(h), 0, 10, 5, 0) == _lib.OVX_ERR_INVALID
+    assert lib.ovx_plan_destroy(None) == _lib.OVX_ERR_INVALID
+    assert lib.ovx_fftm_out_of_place(None, None, 0, 0, None, 0, 0, 0, 0, None) == _lib.OVX_ERR_INVALID
+    with pytest.raises(_lib.OvxInvalid):
+        _lib.check(lib.ovx_plan_destroy(None))
