@@ -1,0 +1,203 @@
+// ovxhip.hpp -- C++ host-side mirror of the reference objects on the
+// fast-convolution path, over the C-ABI of include/ovxhip.h.
+//
+// Error mapping follows the reference's conventions
+// (src/ovxx/support.hpp:138-142 unimplemented, std::bad_alloc on plan
+// failure src/ovxx/fftw/fftw.cpp:97-102, std::runtime_error otherwise):
+//   OVX_ERR_UNSUPPORTED -> ovxhip::unimplemented (a std::runtime_error)
+//   OVX_ERR_NOMEM       -> std::bad_alloc
+//   OVX_ERR_INVALID     -> std::invalid_argument
+//   anything else       -> ovxhip::error (a std::runtime_error)
+//
+// Header-only; link with libovxhip.so.  Data pointers are device pointers
+// unless the object is created with OVX_FLAG_HOST_POINTERS.
+#ifndef OVXHIP_HPP
+#define OVXHIP_HPP
+
+#include <complex>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../ovxhip.h"
+
+namespace ovxhip {
+
+class error : public std::runtime_error {
+public:
+  error(int code, std::string const &msg) : std::runtime_error(msg), code_(code) {}
+  int code() const { return code_; }
+
+private:
+  int code_;
+};
+
+class unimplemented : public error {
+public:
+  explicit unimplemented(std::string const &msg) : error(OVX_ERR_UNSUPPORTED, msg) {}
+};
+
+inline void check(int status)
+{
+  if (status == OVX_OK) return;
+  std::string msg = ovx_last_error();
+  switch (status) {
+  case OVX_ERR_UNSUPPORTED: throw unimplemented(msg);
+  case OVX_ERR_NOMEM: throw std::bad_alloc();
+  case OVX_ERR_INVALID: throw std::invalid_argument(msg);
+  default: throw error(status, msg);
+  }
+}
+
+typedef std::complex<float> cscalar;
+
+// Owns one plan handle.
+class plan_holder {
+public:
+  plan_holder() = default;
+  plan_holder(plan_holder const &) = delete;
+  plan_holder &operator=(plan_holder const &) = delete;
+  ~plan_holder()
+  {
+    if (h_) ovx_plan_destroy(h_);
+  }
+  ovx_plan get() const { return h_; }
+  ovx_plan *out() { return &h_; }
+  char const *name() const { return ovx_plan_name(h_); }
+
+private:
+  ovx_plan h_ = nullptr;
+};
+
+// vsip::Fftm<complex<float>, complex<float>, A, D> (fft.hpp:134-161) with the
+// backend operations of fftm_backend (backend.hpp:696-784).  A: OVX_ROW or
+// OVX_COL; D: OVX_FFT_FWD or OVX_FFT_INV.
+template <int A, int D>
+class Fftm {
+public:
+  Fftm(long rows, long cols, float scale, unsigned flags = 0) : rows_(rows), cols_(cols)
+  {
+    check(ovx_fftm_plan_create(p_.out(), rows, cols, A, D, scale, flags));
+  }
+  // the backend scales inside the kernel (workspace.hpp:474 skips its pass)
+  static bool supports_scale() { return true; }
+  void out_of_place(cscalar const *in, long in_s0, long in_s1, cscalar *out, long out_s0, long out_s1,
+                    long rows, long cols, void *stream = nullptr)
+  {
+    check(ovx_fftm_out_of_place(p_.get(), in, in_s0, in_s1, out, out_s0, out_s1, rows, cols, stream));
+  }
+  void in_place(cscalar *inout, long s0, long s1, long rows, long cols, void *stream = nullptr)
+  {
+    check(ovx_fftm_in_place(p_.get(), inout, s0, s1, rows, cols, stream));
+  }
+  // dense row-major (A = row) or column-major (A = col) convenience call
+  void operator()(cscalar const *in, cscalar *out, void *stream = nullptr)
+  {
+    if (A == OVX_ROW) out_of_place(in, cols_, 1, out, cols_, 1, rows_, cols_, stream);
+    else out_of_place(in, 1, rows_, out, 1, rows_, rows_, cols_, stream);
+  }
+  char const *name() const { return p_.name(); }
+
+private:
+  long rows_, cols_;
+  plan_holder p_;
+};
+
+// vsip::Fft<const_Vector, complex<float>, complex<float>, D> (fft.hpp:75-103).
+template <int D>
+class Fft {
+public:
+  Fft(long n, float scale, unsigned flags = 0) : n_(n) { check(ovx_fft_plan_create(p_.out(), n, D, scale, flags)); }
+  static bool supports_scale() { return true; }
+  void out_of_place(cscalar const *in, long in_s, cscalar *out, long out_s, void *stream = nullptr)
+  {
+    check(ovx_fft_out_of_place(p_.get(), in, in_s, out, out_s, n_, stream));
+  }
+  void in_place(cscalar *inout, long s, void *stream = nullptr)
+  {
+    check(ovx_fft_in_place(p_.get(), inout, s, n_, stream));
+  }
+  char const *name() const { return p_.name(); }
+
+private:
+  long n_;
+  plan_holder p_;
+};
+
+// out = inv_fftm(vmmul<0>(w, fwd_fftm(in))) with the inverse scale, fused
+// (tests/fastconv.cpp:181-213, benchmarks/fastconv.cpp:740-849).
+class Fastconv {
+public:
+  Fastconv(long rows, long n, float scale, unsigned flags = 0) : rows_(rows), n_(n)
+  {
+    check(ovx_fastconv_plan_create(p_.out(), rows, n, scale, flags));
+  }
+  // w: spectrum vector (ldw = 0) or coefficient matrix (row stride ldw)
+  void operator()(cscalar const *x, long ldx, cscalar const *w, long ldw, cscalar *y, long ldy, long rows,
+                  void *stream = nullptr)
+  {
+    check(ovx_fastconv_exec(p_.get(), x, ldx, w, ldw, y, ldy, rows, stream));
+  }
+  void operator()(cscalar const *x, cscalar const *w, cscalar *y, void *stream = nullptr)
+  {
+    (*this)(x, n_, w, 0, y, n_, rows_, stream);
+  }
+  char const *name() const { return p_.name(); }
+
+private:
+  long rows_, n_;
+  plan_holder p_;
+};
+
+// vsip::Correlation<const_Vector, R, complex<float>> (corr.hpp:55-122).
+class Correlation {
+public:
+  Correlation(int support, long ref_size, long input_size, unsigned flags = 0)
+    : support_(support), m_(ref_size), n_(input_size)
+  {
+    check(ovx_corr_plan_create(p_.out(), support, ref_size, input_size, flags));
+  }
+  long reference_size() const { return m_; }
+  long input_size() const { return n_; }
+  long output_size() const { return ovx_corr_output_size(support_, m_, n_); }
+  int support() const { return support_; }
+  void operator()(int bias, cscalar const *ref, cscalar const *in, cscalar *out, void *stream = nullptr)
+  {
+    check(ovx_corr_exec(p_.get(), bias, ref, in, out, stream));
+  }
+
+private:
+  int support_;
+  long m_, n_;
+  plan_holder p_;
+};
+
+// vsip::Convolution<const_Vector, S, R, complex<float>> (conv.hpp:55-108).
+class Convolution {
+public:
+  Convolution(int symmetry, int support, cscalar const *coeff, long coeff_size, long input_size,
+              long decimation = 1, unsigned flags = 0)
+    : support_(support), n_(input_size), d_(decimation)
+  {
+    m_ = symmetry == OVX_SYM_EVEN_LEN_ODD ? 2 * coeff_size - 1
+         : symmetry == OVX_SYM_EVEN_LEN_EVEN ? 2 * coeff_size : coeff_size;
+    check(ovx_conv_plan_create(p_.out(), symmetry, support, coeff, coeff_size, input_size, decimation, flags));
+  }
+  long kernel_size() const { return m_; }
+  long input_size() const { return n_; }
+  long output_size() const { return ovx_conv_output_size(support_, m_, n_, d_); }
+  long decimation() const { return d_; }
+  void operator()(cscalar const *in, cscalar *out, void *stream = nullptr)
+  {
+    check(ovx_conv_exec(p_.get(), in, out, stream));
+  }
+
+private:
+  int support_;
+  long m_, n_, d_;
+  plan_holder p_;
+};
+
+} // namespace ovxhip
+
+#endif

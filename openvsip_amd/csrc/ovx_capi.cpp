@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/ovxhip.h"
+#include "ovx_fourstep.hpp"
 #include "ovx_internal.hpp"
 #include "ovx_plan.hpp"
 
@@ -93,18 +94,59 @@ int LineFFT::init(long n_, int variant)
     kind = LDS;
     return make_row_twiddles(rc, tw);
   }
+  if (n > kMaxRowN && (n & (n - 1)) == 0) {
+    // four-step: n1 <= n2, both with compiled line kernels
+    int lg = 0;
+    while ((1l << lg) < n) ++lg;
+    n1 = 1l << (lg / 2);
+    n2 = n / n1;
+    if (!fs_length_supported(n1) || !fs_length_supported(n2))
+      return fail(OVX_ERR_UNSUPPORTED, "transform length %ld is not supported by the gfx950 backend", n);
+    if (!row_config((int)n1, 0, &rc1) || !row_config((int)n2, 0, &rc2))
+      return fail(OVX_ERR_UNSUPPORTED, "no line configuration for %ld x %ld", n1, n2);
+    int st;
+    if ((st = make_row_twiddles(rc1, tw1)) || (st = make_row_twiddles(rc2, tw2))) return st;
+    sbits = (lg + 1) / 2;
+    long lo = 1l << sbits, hi = (n + lo - 1) / lo;
+    if ((st = tlo.alloc(lo * 8)) || (st = thi.alloc(hi * 8))) return st;
+    hipError_t e = launch_unit_roots(n, lo, (float2 *)tlo.p, nullptr);
+    if (e == hipSuccess) e = launch_unit_roots(n / lo, hi, (float2 *)thi.p, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hip_fail(e, "four-step twiddles");
+    kind = FOURSTEP;
+    return OVX_OK;
+  }
   return fail(OVX_ERR_UNSUPPORTED, "transform length %ld is not supported by the gfx950 backend", n);
+}
+
+static FsArgs fs_args(const LineFFT &f)
+{
+  FsArgs a{};
+  a.n1 = f.n1;
+  a.n2 = f.n2;
+  a.nbig = f.n;
+  a.tw1 = (const float2 *)f.tw1.p;
+  a.tw2 = (const float2 *)f.tw2.p;
+  a.tlo = (const float2 *)f.tlo.p;
+  a.thi = (const float2 *)f.thi.p;
+  a.sbits = f.sbits;
+  a.scale = 1.0f;
+  return a;
 }
 
 const char *LineFFT::name() const
 {
   switch (kind) {
   case LDS: return "lds-resident";
+  case FOURSTEP: return "four-step";
   default: return "none";
   }
 }
 
-size_t LineFFT::device_bytes() const { return tw.bytes; }
+size_t LineFFT::device_bytes() const
+{
+  return tw.bytes + tw1.bytes + tw2.bytes + tlo.bytes + thi.bytes + work.bytes + wp.bytes;
+}
 
 int LineFFT::transform(int dir, const float2 *in, long ld_in, float2 *out, long ld_out, long lines, float scale,
                        hipStream_t s) const
@@ -115,6 +157,26 @@ int LineFFT::transform(int dir, const float2 *in, long ld_in, float2 *out, long 
       return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
     hipError_t e = launch_row_fft(rc, dir, in, ld_in, out, ld_out, lines, (const float2 *)tw.p, scale, s);
     if (e != hipSuccess) return hip_fail(e, "row fft launch");
+    return OVX_OK;
+  }
+  if (kind == FOURSTEP) {
+    if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step transform");
+    int st = work.alloc((size_t)lines * n * 8);
+    if (st) return st;
+    FsArgs a = fs_args(*this);
+    a.in = in;
+    a.ld_in = ld_in;
+    a.out = (float2 *)work.p;
+    a.ld_out = n;
+    hipError_t e = launch_fs_col(dir < 0 ? 0 : 2, a, lines, s);
+    if (e != hipSuccess) return hip_fail(e, "four-step column pass");
+    a.in = (const float2 *)work.p;
+    a.ld_in = n;
+    a.out = out;
+    a.ld_out = ld_out;
+    a.scale = scale;
+    e = launch_fs_row(dir < 0 ? 1 : 2, a, lines, s);
+    if (e != hipSuccess) return hip_fail(e, "four-step row pass");
     return OVX_OK;
   }
   return fail(OVX_ERR_UNSUPPORTED, "no transform path");
@@ -130,6 +192,27 @@ int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, floa
       return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
     hipError_t e = launch_row_fastconv(rc, x, ldx, w, ldw, y, ldy, lines, (const float2 *)tw.p, scale, s);
     if (e != hipSuccess) return hip_fail(e, "fastconv launch");
+    return OVX_OK;
+  }
+  if (kind == FOURSTEP) {
+    if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step fastconv");
+    long wrows = ldw ? lines : 1;
+    int st = wp.alloc((size_t)wrows * n * 8);
+    if (st) return st;
+    hipError_t e = launch_fs_permute(w, ldw, (float2 *)wp.p, n1, n2, wrows, scale, s);
+    if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
+    FsArgs a = fs_args(*this);
+    a.wp = (const float2 *)wp.p;
+    a.ldwp = ldw ? n : 0;
+    a.in = x;
+    a.ld_in = ldx;
+    a.out = y;
+    a.ld_out = ldy;
+    if ((e = launch_fs_col(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
+    a.in = y;
+    a.ld_in = ldy;
+    if ((e = launch_fs_row(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step row pass");
+    if ((e = launch_fs_col(1, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
     return OVX_OK;
   }
   return fail(OVX_ERR_UNSUPPORTED, "no fastconv path");
@@ -184,10 +267,12 @@ extern "C" int ovx_init(int device)
 
 extern "C" const char *ovx_last_error(void) { return g_err.c_str(); }
 extern "C" const char *ovx_version(void) { return "ovxhip 0.1 (gfx950)"; }
-extern "C" long ovx_max_length(void) { return kMaxRowN; }
+extern "C" long ovx_max_length(void) { return 2048l * 2048l; }
 extern "C" int ovx_supported_length(long n)
 {
-  return (n >= 1 && n <= kMaxRowN && row_variants((int)n) > 0) ? 1 : 0;
+  if (n >= 1 && n <= kMaxRowN) return row_variants((int)n) > 0 ? 1 : 0;
+  if (n > kMaxRowN && n <= ovx_max_length() && (n & (n - 1)) == 0) return 1;
+  return 0;
 }
 
 // ---------------------------------------------------------------- Fftm

@@ -28,10 +28,16 @@ struct DevBuf {
 // A line transform of length n: picks the kernel path (LDS-resident rows,
 // or the two-pass four-step for longer lines) and owns its twiddles.
 struct LineFFT {
-  enum Kind { NONE, LDS } kind = NONE;
+  enum Kind { NONE, LDS, FOURSTEP } kind = NONE;
   long n = 0;
   RowConfig rc;
   DevBuf tw;
+  // two-pass four-step (n = n1 * n2)
+  long n1 = 0, n2 = 0;
+  int sbits = 0;
+  RowConfig rc1, rc2;
+  DevBuf tw1, tw2, tlo, thi;
+  mutable DevBuf work, wp; // Fftm workspace, permuted spectrum
 
   int init(long n, int variant);
   const char *name() const;

@@ -249,3 +249,47 @@ def test_convolution_vs_oracle(ovx, symmetry, support, d):
     exp = o.conv64(support, k, x, d)
     assert y.shape == exp.shape
     assert o.rel_l2(y, exp) < REL_L2
+
+
+# ---- rows longer than LDS: two-pass four-step ----------------------------------
+
+@pytest.mark.parametrize("n", [32768, 65536, 131072, 262144])
+@pytest.mark.parametrize("direction", ["fwd", "inv"])
+def test_fftm_fourstep_vs_fp64(ovx, n, direction):
+    say("fftm four-step", n, direction)
+    rows = 3
+    rng = np.random.default_rng(n + 3)
+    x = cg(rng, (rows, n))
+    d = ovx.FFT_FWD if direction == "fwd" else ovx.FFT_INV
+    scale = 1.0 if direction == "fwd" else 1.0 / n
+    f = ovx.Fftm(rows, n, ovx.ROW, d, scale)
+    assert "four-step" in f.name
+    y = host(f(dev(x)))
+    ref = o.fftm64(x.astype(np.complex128), o.FWD if direction == "fwd" else o.INV, scale)
+    err = o.rel_l2(y, ref)
+    say("  rel_l2", err)
+    assert err < REL_L2
+
+
+@pytest.mark.parametrize("n", [65536, 262144])
+def test_fastconv_fourstep_vs_fp64(ovx, n):
+    rows = 64 if n == 262144 else 20
+    say("fastconv four-step", rows, n)
+    x, w, _, _ = wl.radar(rows, n, seed=1)
+    fc = ovx.Fastconv(rows, n)
+    assert "four-step" in fc.name
+    y = host(fc(dev(x), dev(w)))
+    ref = o.fastconv64(x, w)
+    err = o.rel_l2(y, ref)
+    say("  rel_l2", err, "db", o.diff_db(y, ref))
+    assert err < REL_L2
+    assert o.diff_db(y, ref) < DB
+
+
+def test_fastconv_fourstep_coefficient_matrix(ovx):
+    rows, n = 3, 32768
+    rng = np.random.default_rng(21)
+    x, c = cg(rng, (rows, n)), cg(rng, (rows, n))
+    fc = ovx.Fastconv(rows, n)
+    y = host(fc(dev(x), dev(c)))
+    assert o.rel_l2(y, o.fastconv64(x, c)) < REL_L2
