@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bound on the CPU baseline sample (seconds of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", type=int, default=None, help="force a compiled kernel variant")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-jobs measurement")
+    ap.add_argument("--batch-steps", type=int, default=10)
     return ap.parse_args()
 
 
@@ -62,6 +65,72 @@ def cpu_baseline(x, w, seconds):
     }
 
 
+def batch_section(args):
+    """512 uneven fastconv jobs (seed 3: N drawn from {512..16384, 65536,
+    3072, 6144, 12288}, rows U[16, 256]) in one persistent tile-queue launch
+    replayed from a captured hipGraph, against one standalone launch per job.
+    Inputs are generated on the device (seeded torch generator); three jobs
+    are checked against the FP64 oracle."""
+    import numpy as np
+    import torch
+
+    import openvsip_amd as ovx
+    import oracle
+    from openvsip_amd import workloads as wl
+
+    spec = wl.batch_jobs(seed=3)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(3)
+    reps, jobs = {}, []
+    for n in sorted({n for n, _ in spec}):
+        reps[n] = ovx.Replica(n, wl.lfm_chirp(n, n // 8).astype(np.complex64))
+    b = ovx.Batch()
+    alg = 0
+    for n, rows in spec:
+        x = torch.randn(rows, n, dtype=torch.complex64, device="cuda", generator=gen)
+        y = torch.empty_like(x)
+        b.add(x, reps[n], y)
+        jobs.append((n, rows, x, y))
+        alg += 2 * rows * n * 8 + n * 8
+    b.commit()
+    for _ in range(2):
+        b()
+        b.plain()
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.batch_steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.batch_steps / 1e3
+
+    t_batch = timed(b)
+    t_plain = timed(b.plain)
+    b()
+    torch.cuda.synchronize()
+    checks = []
+    for k in sorted(range(len(jobs)), key=lambda i: jobs[i][1])[:3]:
+        n, rows, x, y = jobs[k]
+        chirp = wl.lfm_chirp(n, n // 8).astype(np.complex64)
+        w = wl.replica_spectrum(chirp.astype(np.complex128), n)
+        checks.append(oracle.rel_l2(y.cpu().numpy(), oracle.fastconv64(x.cpu().numpy(), w)))
+    total_rows = sum(r for _, r in spec)
+    return {
+        "workload": "512 fastconv jobs, seed 3, N in {512,1024,2048,4096,8192,16384,65536,3072,6144,12288}, "
+                    "rows U[16,256]; replica LFM chirp (BT = N/8) resident per N",
+        "jobs": len(spec), "rows": total_rows, "tiles": b.tiles, "launches_per_replay": b.launches,
+        "batched_s": t_batch, "plain_s": t_plain,
+        "batched_rows_per_s": total_rows / t_batch, "plain_rows_per_s": total_rows / t_plain,
+        "batched_jobs_per_s": len(spec) / t_batch, "speedup_vs_plain": t_plain / t_batch,
+        "batched_alg_GBs": alg / t_batch / 1e9, "plain_alg_GBs": alg / t_plain / 1e9,
+        "rel_l2_checked_jobs": checks,
+    }
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -82,15 +151,16 @@ def main():
     R, N = args.rows, args.n
     x, w, _, _ = wl.radar(R, N, seed=1)
     xd = torch.from_numpy(x).cuda()
-    wd = torch.from_numpy(w).cuda()
     yd = torch.empty_like(xd)
-    fc = ovx.Fastconv(R, N)
+    # the replica spectrum is resident: transformed once, prepared once in the
+    # kernel's layout (the serving path); the timed steps read X, write Y.
+    rep = ovx.Replica(N, w, is_spectrum=True)
+    fc = ovx.Fastconv(R, N, variant=args.variant)
     lib = _lib.lib()
     stream = torch.cuda.current_stream()
     sptr = ctypes.c_void_p(stream.cuda_stream)
-    args_c = (fc._h, ctypes.c_void_p(xd.data_ptr()), N, ctypes.c_void_p(wd.data_ptr()), 0,
-              ctypes.c_void_p(yd.data_ptr()), N, R, sptr)
-    fn = lib.ovx_fastconv_exec
+    args_c = (fc._h, ctypes.c_void_p(xd.data_ptr()), N, rep.id, ctypes.c_void_p(yd.data_ptr()), N, R, sptr)
+    fn = lib.ovx_fastconv_exec_replica
 
     for _ in range(args.warmup):
         _lib.check(fn(*args_c))
@@ -140,6 +210,8 @@ def main():
         "gflops": flops / sec / 1e9,
         "parity": {"rel_l2_vs_fp64": rel, "rows_checked": len(idx)},
     }
+    if not args.no_batch:
+        out["batch"] = batch_section(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, w, args.cpu_seconds)
     print(json.dumps(out), flush=True)

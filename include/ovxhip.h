@@ -62,6 +62,10 @@ typedef enum {
 #define OVX_FLAG_HOST_POINTERS 0x1u /* data pointers are host memory; the call
                                        stages through device buffers and
                                        returns after the result is back    */
+/* Kernel variant of a transform length (0 = the default or the autotuner's
+   cached choice; v >= 1 forces compiled variant v-1, see
+   ovx_length_variants). */
+#define OVX_FLAG_VARIANT(v) (((unsigned)(v) & 0xffu) << 8)
 
 /* Correlation / convolution enums (src/vsip/impl/signal/types.hpp). */
 #define OVX_SUPPORT_FULL 0
@@ -86,6 +90,8 @@ const char *ovx_version(void);
 long ovx_max_length(void);
 /* 1 if `n` is a transform length this backend handles, else 0. */
 int ovx_supported_length(long n);
+/* Number of compiled kernel variants for transform length n. */
+int ovx_length_variants(long n);
 
 /* ---- Fftm (multiple 1-D transforms) ----------------------------------- */
 /* Create a plan for a rows x cols Fftm along `axis` in `direction`, result
@@ -145,6 +151,83 @@ int ovx_replica_create(long n, const void *replica, long m, int is_spectrum,
 int ovx_replica_destroy(int id);
 /* Device pointer to the spectrum (n complex<float>), for ovx_fastconv_exec. */
 int ovx_replica_spectrum(int id, const void **spectrum, long *n);
+/* Fused fastconv against a resident replica: the spectrum was prepared once,
+   at ovx_replica_create, in the layout the plan's kernel reads (permuted,
+   with the 1/n scale folded in), so a call reads X, writes Y and reads the
+   prepared spectrum -- nothing else.  Same result as ovx_fastconv_exec with
+   the replica's spectrum.  Device pointers only. */
+int ovx_fastconv_exec_replica(ovx_plan plan, const void *x, long ldx, int replica,
+                              void *y, long ldy, long rows, void *stream);
+
+/* ---- batches of uneven fastconv jobs ------------------------------------ */
+/* No reference counterpart: the reference runs one Fftm object per shape
+   (benchmarks/fastconv.cpp loops one size at a time).  A batch packs jobs of
+   different n and row counts (each against a resident replica) into one
+   persistent-kernel launch over an (instance, rows) tile queue (up to three
+   launches when jobs need the four-step path), captured once into a
+   hipGraph at commit and replayed by every exec.  Each job's result is
+   bitwise the same as ovx_fastconv_exec_replica on the same job. */
+typedef struct ovx_batch_s *ovx_batch;
+int ovx_batch_create(ovx_batch *batch);
+int ovx_batch_add_fastconv(ovx_batch batch, long n, long rows, const void *x,
+                           long ldx, int replica, void *y, long ldy);
+int ovx_batch_commit(ovx_batch batch, void *stream);
+int ovx_batch_exec(ovx_batch batch, void *stream);
+/* The plain path it is compared with: one standalone launch per job. */
+int ovx_batch_exec_plain(ovx_batch batch, void *stream);
+long ovx_batch_tiles(ovx_batch batch);
+int ovx_batch_launches(ovx_batch batch);
+int ovx_batch_destroy(ovx_batch batch);
+
+/* ---- continuous-batching server ----------------------------------------- */
+/* No reference counterpart: serving pulse-compression jobs, standalone
+   forward Fftms and correlations as they arrive.  A scheduler thread admits
+   queued requests (in arrival order, while their device working sets fit the
+   HBM budget) into a free slot -- a stream with a hipGraph of persistent
+   tile-queue launches captured at creation -- and replays it; requests that
+   arrive meanwhile go into the next free slot, so batches overlap on the
+   device.  Data pointers are device pointers owned by the caller and must
+   stay valid until the request completes.  plain != 0 runs every request as
+   its own standalone launch instead (the per-request path). */
+typedef struct ovx_server_s *ovx_server;
+typedef struct {
+  int slots;          /* batches in flight at once (>= 1)                    */
+  int max_jobs;       /* requests per batch                                  */
+  long max_tiles;     /* tile-table capacity per queue (0 = 65536)           */
+  long hbm_budget_mb; /* admission budget for request working sets           */
+  long workspace_mb;  /* four-step Fftm workspace per slot                   */
+  int plain;          /* 1: one standalone launch per request                */
+} ovx_server_config;
+
+#define OVX_REQ_FASTCONV 1 /* ref = replica id; y = fastconv of x            */
+#define OVX_REQ_FFTM 2     /* forward Fftm by row, scale 1                    */
+#define OVX_REQ_CORR 3     /* ref = correlation reference id; x has length n */
+typedef struct {
+  double t_arrival;   /* seconds from the start of a trace (run_trace only)  */
+  int kind;
+  long n, rows;
+  const void *x;
+  long ldx;
+  void *y;
+  long ldy;
+  int ref;
+  int bias;           /* correlation: OVX_BIASED / OVX_UNBIASED             */
+} ovx_request;
+
+int ovx_server_create(ovx_server *server, const ovx_server_config *cfg);
+int ovx_server_submit(ovx_server server, const ovx_request *req, unsigned long *ticket);
+/* Wait for a request; its latency (completion - submission) in seconds. */
+int ovx_server_wait(ovx_server server, unsigned long ticket, double timeout_s,
+                    double *latency_s);
+/* Submit reqs[i] at t_arrival[i] (real time) and wait for all; per-request
+   latencies and the time from the first arrival to the last completion. */
+int ovx_server_run_trace(ovx_server server, const ovx_request *reqs, long count,
+                         double *latency_s, double *makespan_s);
+int ovx_server_destroy(ovx_server server);
+/* A correlation reference prepared once for serving: reference of length m
+   (host data) against inputs of length n with the given support. */
+int ovx_corr_reference_create(int support, long m, long n, const void *ref, int *id);
+int ovx_corr_reference_destroy(int id);
 
 /* ---- Correlation (FFT-based, overlap-save) ----------------------------- */
 /* vsip::Correlation<const_Vector, support, complex<float>> of a reference of

@@ -1,0 +1,195 @@
+// vsip_plugin.hpp -- registers the gfx950 backend with OpenVSIP's dispatcher.
+//
+// Include this header in a translation unit that includes <vsip/signal.hpp>
+// (and therefore a configured OpenVSIP tree) and link libovxhip.so.  It adds
+// Evaluator specialisations on be::user, which is first in every relevant
+// dispatch list:
+//   op::fft   list  src/ovxx/signal/fft.hpp:35-49
+//   op::fftm  list  src/ovxx/signal/fft.hpp:51-64
+//   assign<2> list  src/ovxx/assign_fwd.hpp:49-62
+// so existing vsip::Fft / vsip::Fftm / vmmul code selects the HIP kernels
+// without source changes; rt_valid() returns false for a shape the backend
+// does not handle, and the dispatcher falls through to the next backend
+// (src/ovxx/dispatcher/dispatcher.hpp:96-130).
+//
+// Data reach the backend through OpenVSIP's DDA as host pointers, so the
+// plans are created with OVX_FLAG_HOST_POINTERS (the library stages the data
+// through device buffers); an application that keeps its data on the device
+// calls include/ovxhip.h directly.
+//
+// This header compiles only against an OpenVSIP tree; it is not built by this
+// repository's tests.
+#ifndef OVXHIP_VSIP_PLUGIN_HPP
+#define OVXHIP_VSIP_PLUGIN_HPP
+
+#include <memory>
+
+#include <ovxx/dispatch.hpp>
+#include <ovxx/dda.hpp>
+#include <ovxx/expr/unary.hpp>
+#include <ovxx/expr/vmmul.hpp>
+#include <ovxx/signal/fft.hpp>
+#include <ovxx/signal/fft/backend.hpp>
+#include <ovxx/signal/fft/functor.hpp>
+
+#include "ovxhip.hpp"
+
+namespace ovxx
+{
+namespace ovxhip_backend
+{
+using vsip::complex;
+typedef complex<float> ctype;
+
+// fftm_backend<c,c,A,D> on the gfx950 kernels (src/ovxx/signal/fft/backend.hpp:696-784).
+template <int A, int D>
+class fftm : public signal::fft::fftm_backend<ctype, ctype, A, D>
+{
+  static int const dir = D == fft_fwd ? OVX_FFT_FWD : OVX_FFT_INV;
+  static int const axis = A == vsip::col ? OVX_COL : OVX_ROW;
+
+public:
+  fftm(vsip::Domain<2> const &dom, float scale)
+    : plan_(dom[0].size(), dom[1].size(), scale, OVX_FLAG_HOST_POINTERS), scale_(scale)
+  {}
+  virtual char const *name() { return "ovxhip::fftm"; }
+  // the kernels scale: workspace.hpp:474 skips its separate pass
+  virtual bool supports_scale() { return true; }
+  // default query_layout (dense, unit stride along the transform axis,
+  // interleaved complex) is what the kernels take
+  virtual void in_place(ctype *inout, stride_type s0, stride_type s1, length_type rows, length_type cols)
+  {
+    plan_.in_place(reinterpret_cast<ovxhip::cscalar *>(inout), s0, s1, rows, cols);
+  }
+  virtual void out_of_place(ctype *in, stride_type in_s0, stride_type in_s1, ctype *out, stride_type out_s0,
+                            stride_type out_s1, length_type rows, length_type cols)
+  {
+    plan_.out_of_place(reinterpret_cast<ovxhip::cscalar const *>(in), in_s0, in_s1,
+                       reinterpret_cast<ovxhip::cscalar *>(out), out_s0, out_s1, rows, cols);
+  }
+  float scale() const { return scale_; }
+
+private:
+  ovxhip::Fftm<axis, dir> plan_;
+  float scale_;
+};
+
+// fft_backend<1,c,c,S> (src/ovxx/signal/fft/backend.hpp:131-204).
+template <int S>
+class fft : public signal::fft::fft_backend<1, ctype, ctype, S>
+{
+  static int const dir = S == fft_fwd ? OVX_FFT_FWD : OVX_FFT_INV;
+
+public:
+  fft(vsip::Domain<1> const &dom, float scale) : plan_(dom.size(), scale, OVX_FLAG_HOST_POINTERS) {}
+  virtual char const *name() { return "ovxhip::fft"; }
+  virtual bool supports_scale() { return true; }
+  virtual void in_place(ctype *inout, stride_type s, length_type)
+  {
+    plan_.in_place(reinterpret_cast<ovxhip::cscalar *>(inout), s);
+  }
+  virtual void out_of_place(ctype *in, stride_type in_s, ctype *out, stride_type out_s, length_type)
+  {
+    plan_.out_of_place(reinterpret_cast<ovxhip::cscalar const *>(in), in_s, reinterpret_cast<ovxhip::cscalar *>(out),
+                       out_s);
+  }
+
+private:
+  ovxhip::Fft<dir> plan_;
+};
+
+inline bool length_ok(length_type n) { return ovx_supported_length((long)n) != 0; }
+
+} // namespace ovxx::ovxhip_backend
+
+namespace dispatcher
+{
+// Fftm factory (pattern: share/examples/dispatch/fft.cpp:165-200).
+template <int A, int D, vsip::return_mechanism_type R, unsigned N>
+struct Evaluator<op::fftm<complex<float>, complex<float>, A, D, R, N>, be::user,
+                 std::unique_ptr<signal::fft::fftm_backend<complex<float>, complex<float>, A, D> >(
+                   vsip::Domain<2> const &, float)>
+{
+  static bool const ct_valid = true;
+  static bool rt_valid(vsip::Domain<2> const &dom, float)
+  {
+    return ovxhip_backend::length_ok(A == vsip::col ? dom[0].size() : dom[1].size());
+  }
+  static std::unique_ptr<signal::fft::fftm_backend<complex<float>, complex<float>, A, D> >
+  exec(vsip::Domain<2> const &dom, float scale)
+  {
+    return std::unique_ptr<signal::fft::fftm_backend<complex<float>, complex<float>, A, D> >(
+      new ovxhip_backend::fftm<A, D>(dom, scale));
+  }
+};
+
+// Fft factory: one complex 1-D transform (e.g. the replica spectrum,
+// tests/fastconv.cpp:20-26).
+template <int S, vsip::return_mechanism_type R, unsigned N>
+struct Evaluator<op::fft<1, complex<float>, complex<float>, S, R, N>, be::user,
+                 std::unique_ptr<signal::fft::fft_backend<1, complex<float>, complex<float>, S> >(
+                   vsip::Domain<1> const &, float)>
+{
+  static bool const ct_valid = true;
+  static bool rt_valid(vsip::Domain<1> const &dom, float) { return ovxhip_backend::length_ok(dom.size()); }
+  static std::unique_ptr<signal::fft::fft_backend<1, complex<float>, complex<float>, S> >
+  exec(vsip::Domain<1> const &dom, float scale)
+  {
+    return std::unique_ptr<signal::fft::fft_backend<1, complex<float>, complex<float>, S> >(
+      new ovxhip_backend::fft<S>(dom, scale));
+  }
+};
+
+// The fused fast convolution, LHS = inv_fftm(vmmul<0>(w, fwd_fftm(M))): the
+// by-value expression of tests/fastconv.cpp:200-205 and
+// benchmarks/fastconv.cpp:797-801, which the reference evaluates as four
+// passes through be::rbo_expr.  This evaluator reads w and M through DDA and
+// makes one fused call.  Pattern: share/examples/eval/fft_expression.cpp:48-74;
+// guard: is_fftm_functor (src/ovxx/signal/fft/functor.hpp:130-139).
+template <typename LHS, template <typename> class FI, typename VB, template <typename> class FF, typename MB>
+struct Evaluator<op::assign<2>, be::user,
+                 void(LHS &, expr::Unary<FI, expr::Vmmul<0, VB, expr::Unary<FF, MB> const> const> const &),
+                 typename enable_if<expr::is_fftm_functor<FI<expr::Vmmul<0, VB, expr::Unary<FF, MB> const> > >::value &&
+                                    expr::is_fftm_functor<FF<MB> >::value>::type>
+{
+  static char const *name() { return "be::user ovxhip fused fastconv"; }
+  typedef expr::Unary<FF, MB> fwd_block_type;
+  typedef expr::Vmmul<0, VB, fwd_block_type const> vmmul_block_type;
+  typedef expr::Unary<FI, vmmul_block_type const> RHS;
+  typedef ovxhip_backend::fftm<vsip::row, fft_inv> inv_backend_type;
+  typedef ovxhip_backend::fftm<vsip::row, fft_fwd> fwd_backend_type;
+
+  static bool const ct_valid = is_same<typename LHS::value_type, complex<float> >::value &&
+                               is_same<typename VB::value_type, complex<float> >::value &&
+                               is_same<typename MB::value_type, complex<float> >::value;
+
+  // both transforms must be this backend's row Fftms (their plans hold the
+  // inverse scale) and the rows must fit a supported length
+  static bool rt_valid(LHS &lhs, RHS const &rhs)
+  {
+    inv_backend_type const *inv = dynamic_cast<inv_backend_type const *>(&rhs.operation().backend());
+    fwd_backend_type const *fwd =
+      dynamic_cast<fwd_backend_type const *>(&rhs.arg().get_mblk().operation().backend());
+    return inv && fwd && ovxhip_backend::length_ok(lhs.size(2, 1));
+  }
+
+  static void exec(LHS &lhs, RHS const &rhs)
+  {
+    inv_backend_type const *inv = dynamic_cast<inv_backend_type const *>(&rhs.operation().backend());
+    VB const &w = rhs.arg().get_vblk();
+    MB const &m = rhs.arg().get_mblk().arg();
+    length_type rows = lhs.size(2, 0), cols = lhs.size(2, 1);
+    dda::Data<VB, dda::in> w_data(w);
+    dda::Data<MB, dda::in> m_data(m);
+    dda::Data<LHS, dda::out> out_data(lhs);
+    ovxhip::Fastconv fc(rows, cols, inv->scale(), OVX_FLAG_HOST_POINTERS);
+    fc(reinterpret_cast<ovxhip::cscalar const *>(m_data.ptr()), m_data.stride(0),
+       reinterpret_cast<ovxhip::cscalar const *>(w_data.ptr()), 0,
+       reinterpret_cast<ovxhip::cscalar *>(out_data.ptr()), out_data.stride(0), rows);
+  }
+};
+
+} // namespace ovxx::dispatcher
+} // namespace ovxx
+
+#endif

@@ -5,5 +5,6 @@ package holds its sources (csrc/), the Python ctypes binding used by the
 tests and the benchmark (api.py), and seeded workload generators.
 """
 from ._lib import OvxError, OvxInvalid, OvxLibraryMissing, OvxUnsupported, LIB_PATH  # noqa: F401
-from .api import (COL, FFT_FWD, FFT_INV, ROW, Convolution, Correlation, Fastconv, Fft,  # noqa: F401
-                  Fftm, Replica, init)
+from .api import (COL, FFT_FWD, FFT_INV, ROW, Batch, Convolution, Correlation, Fastconv, Fft,  # noqa: F401
+                  Fftm, Replica, init, variants)
+from .api import CorrReference, Server, corr_request, fastconv_request, fftm_request  # noqa: F401

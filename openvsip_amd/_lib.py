@@ -25,6 +25,10 @@ ROW = 0
 COL = 1
 FLAG_HOST_POINTERS = 0x1
 
+
+def FLAG_VARIANT(v):
+    return (v & 0xFF) << 8
+
 SUPPORT = {"full": 0, "same": 1, "min": 2}
 BIAS = {"biased": 0, "unbiased": 1}
 SYMMETRY = {"nonsym": 0, "sym_even_len_odd": 1, "sym_even_len_even": 2}
@@ -42,6 +46,7 @@ SIGNATURES = {
     "ovx_version": (ctypes.c_char_p, []),
     "ovx_max_length": (_L, []),
     "ovx_supported_length": (_I, [_L]),
+    "ovx_length_variants": (_I, [_L]),
     "ovx_fftm_plan_create": (_I, [_PP, _L, _L, _I, _I, _F, _U]),
     "ovx_fftm_out_of_place": (_I, [_P, _P, _L, _L, _P, _L, _L, _L, _L, _P]),
     "ovx_fftm_in_place": (_I, [_P, _P, _L, _L, _L, _L, _P]),
@@ -53,6 +58,22 @@ SIGNATURES = {
     "ovx_replica_create": (_I, [_L, _P, _L, _I, ctypes.POINTER(_I)]),
     "ovx_replica_destroy": (_I, [_I]),
     "ovx_replica_spectrum": (_I, [_I, _PP, ctypes.POINTER(_L)]),
+    "ovx_fastconv_exec_replica": (_I, [_P, _P, _L, _I, _P, _L, _L, _P]),
+    "ovx_batch_create": (_I, [_PP]),
+    "ovx_batch_add_fastconv": (_I, [_P, _L, _L, _P, _L, _I, _P, _L]),
+    "ovx_batch_commit": (_I, [_P, _P]),
+    "ovx_batch_exec": (_I, [_P, _P]),
+    "ovx_batch_exec_plain": (_I, [_P, _P]),
+    "ovx_batch_tiles": (_L, [_P]),
+    "ovx_batch_launches": (_I, [_P]),
+    "ovx_batch_destroy": (_I, [_P]),
+    "ovx_server_create": (_I, [_PP, ctypes.c_void_p]),
+    "ovx_server_submit": (_I, [_P, ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulong)]),
+    "ovx_server_wait": (_I, [_P, ctypes.c_ulong, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
+    "ovx_server_run_trace": (_I, [_P, ctypes.c_void_p, _L, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "ovx_server_destroy": (_I, [_P]),
+    "ovx_corr_reference_create": (_I, [_I, _L, _L, _P, ctypes.POINTER(_I)]),
+    "ovx_corr_reference_destroy": (_I, [_I]),
     "ovx_corr_plan_create": (_I, [_PP, _I, _L, _L, _U]),
     "ovx_corr_output_size": (_L, [_I, _L, _L]),
     "ovx_corr_exec": (_I, [_P, _I, _P, _P, _P, _P]),
@@ -63,6 +84,22 @@ SIGNATURES = {
     "ovx_plan_name": (ctypes.c_char_p, [_P]),
     "ovx_plan_device_bytes": (ctypes.c_size_t, [_P]),
 }
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("max_jobs", ctypes.c_int), ("max_tiles", ctypes.c_long),
+                ("hbm_budget_mb", ctypes.c_long), ("workspace_mb", ctypes.c_long), ("plain", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("t_arrival", ctypes.c_double), ("kind", ctypes.c_int), ("n", ctypes.c_long),
+                ("rows", ctypes.c_long), ("x", ctypes.c_void_p), ("ldx", ctypes.c_long), ("y", ctypes.c_void_p),
+                ("ldy", ctypes.c_long), ("ref", ctypes.c_int), ("bias", ctypes.c_int)]
+
+
+REQ_FASTCONV = 1
+REQ_FFTM = 2
+REQ_CORR = 3
 
 
 class OvxLibraryMissing(RuntimeError):

@@ -70,13 +70,23 @@ class _Plan:
         return lib().ovx_plan_device_bytes(self._h)
 
 
+def variants(n):
+    """Number of compiled kernel variants for transform length n."""
+    return lib().ovx_length_variants(n)
+
+
+def _flags(variant):
+    return 0 if variant is None else _lib.FLAG_VARIANT(variant + 1)
+
+
 class Fftm(_Plan):
     """Multiple FFTs along `axis` (ROW: each row) with in-kernel scaling."""
 
-    def __init__(self, rows, cols, axis=ROW, direction=FFT_FWD, scale=1.0):
+    def __init__(self, rows, cols, axis=ROW, direction=FFT_FWD, scale=1.0, variant=None):
         super().__init__()
         self.rows, self.cols, self.axis, self.direction, self.scale = rows, cols, axis, direction, scale
-        check(lib().ovx_fftm_plan_create(ctypes.byref(self._h), rows, cols, axis, direction, scale, 0))
+        check(lib().ovx_fftm_plan_create(ctypes.byref(self._h), rows, cols, axis, direction, scale,
+                                         _flags(variant)))
 
     def __call__(self, x, out=None):
         torch = _torch()
@@ -114,11 +124,11 @@ class Fft(_Plan):
 class Fastconv(_Plan):
     """Y = scale * IFFT_row( W (.) FFT_row(X) ) in one fused pass."""
 
-    def __init__(self, rows, n, scale=None):
+    def __init__(self, rows, n, scale=None, variant=None):
         super().__init__()
         self.rows, self.n = rows, n
         self.scale = 1.0 / n if scale is None else scale
-        check(lib().ovx_fastconv_plan_create(ctypes.byref(self._h), rows, n, self.scale, 0))
+        check(lib().ovx_fastconv_plan_create(ctypes.byref(self._h), rows, n, self.scale, _flags(variant)))
 
     def __call__(self, x, w, out=None, rows=None):
         torch = _torch()
@@ -132,8 +142,16 @@ class Fastconv(_Plan):
                                       rows, _stream()))
         return out
 
-    def exec_raw(self, x_ptr, ldx, w_ptr, ldw, y_ptr, ldy, rows, stream):
-        return lib().ovx_fastconv_exec(self._h, x_ptr, ldx, w_ptr, ldw, y_ptr, ldy, rows, stream)
+    def with_replica(self, x, replica, out=None, rows=None):
+        """Fastconv against a resident replica (spectrum prepared once)."""
+        torch = _torch()
+        _check_tensor(x, "input")
+        if out is None:
+            out = torch.empty_like(x)
+        rows = x.shape[0] if rows is None else rows
+        check(lib().ovx_fastconv_exec_replica(self._h, _ptr(x), x.stride(0), replica.id, _ptr(out),
+                                              out.stride(0), rows, _stream()))
+        return out
 
 
 class Replica:
@@ -146,6 +164,10 @@ class Replica:
         self._id = ctypes.c_int()
         check(lib().ovx_replica_create(n, r.ctypes.data_as(ctypes.c_void_p), r.size, int(bool(is_spectrum)),
                                        ctypes.byref(self._id)))
+
+    @property
+    def id(self):
+        return self._id.value
 
     def spectrum_ptr(self):
         p = ctypes.c_void_p()
@@ -225,3 +247,136 @@ class Convolution(_Plan):
 
 def init(device=0):
     check(lib().ovx_init(device))
+
+
+class Batch:
+    """Uneven fastconv jobs packed into one persistent launch (graph replay)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        check(lib().ovx_batch_create(ctypes.byref(self._h)))
+        self._keep = []
+
+    def add(self, x, replica, out):
+        _check_tensor(x, "input")
+        _check_tensor(out, "output")
+        rows, n = x.shape
+        check(lib().ovx_batch_add_fastconv(self._h, n, rows, _ptr(x), x.stride(0), replica.id, _ptr(out),
+                                           out.stride(0)))
+        self._keep.append((x, replica, out))
+
+    def commit(self):
+        check(lib().ovx_batch_commit(self._h, _stream()))
+
+    def __call__(self):
+        check(lib().ovx_batch_exec(self._h, _stream()))
+
+    def plain(self):
+        """One standalone launch per job (the per-request path)."""
+        check(lib().ovx_batch_exec_plain(self._h, _stream()))
+
+    @property
+    def tiles(self):
+        return lib().ovx_batch_tiles(self._h)
+
+    @property
+    def launches(self):
+        return lib().ovx_batch_launches(self._h)
+
+    def close(self):
+        if self._h:
+            lib().ovx_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CorrReference:
+    """A correlation reference prepared once for serving (H = FFT(reversed ref))."""
+
+    def __init__(self, support, ref, n):
+        import numpy as np
+        r = np.ascontiguousarray(ref, dtype=np.complex64)
+        self.support, self.m, self.n = support, r.size, n
+        self._id = ctypes.c_int()
+        check(lib().ovx_corr_reference_create(_lib.SUPPORT[support], r.size, n, r.ctypes.data_as(ctypes.c_void_p),
+                                              ctypes.byref(self._id)))
+        self.output_size = lib().ovx_corr_output_size(_lib.SUPPORT[support], r.size, n)
+
+    @property
+    def id(self):
+        return self._id.value
+
+    def close(self):
+        if self._id.value:
+            lib().ovx_corr_reference_destroy(self._id)
+            self._id = ctypes.c_int(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fastconv_request(x, replica, y, t_arrival=0.0):
+    rows, n = x.shape
+    return _lib.Request(t_arrival, _lib.REQ_FASTCONV, n, rows, x.data_ptr(), x.stride(0), y.data_ptr(), y.stride(0),
+                        replica.id, 0)
+
+
+def fftm_request(x, y, t_arrival=0.0):
+    rows, n = x.shape
+    return _lib.Request(t_arrival, _lib.REQ_FFTM, n, rows, x.data_ptr(), x.stride(0), y.data_ptr(), y.stride(0), 0, 0)
+
+
+def corr_request(x, reference, y, bias="biased", t_arrival=0.0):
+    return _lib.Request(t_arrival, _lib.REQ_CORR, reference.n, 1, x.data_ptr(), x.numel(), y.data_ptr(), y.numel(),
+                        reference.id, _lib.BIAS[bias])
+
+
+class Server:
+    """Continuous-batching server (ovx_server_*); plain=True runs every request
+    as its own standalone launch."""
+
+    def __init__(self, slots=3, max_jobs=512, max_tiles=65536, hbm_budget_mb=64 * 1024, workspace_mb=1024,
+                 plain=False):
+        cfg = _lib.ServerConfig(1 if plain else slots, 1 if plain else max_jobs, max_tiles, hbm_budget_mb,
+                                workspace_mb, int(bool(plain)))
+        self._h = ctypes.c_void_p()
+        check(lib().ovx_server_create(ctypes.byref(self._h), ctypes.byref(cfg)))
+
+    def submit(self, req):
+        t = ctypes.c_ulong()
+        check(lib().ovx_server_submit(self._h, ctypes.byref(req), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout=60.0):
+        lat = ctypes.c_double()
+        check(lib().ovx_server_wait(self._h, ticket, timeout, ctypes.byref(lat)))
+        return lat.value
+
+    def run_trace(self, reqs):
+        """reqs: list of Request with t_arrival set; returns (latencies, makespan)."""
+        import numpy as np
+        arr = (_lib.Request * len(reqs))(*reqs)
+        lat = np.zeros(len(reqs), dtype=np.float64)
+        mk = ctypes.c_double()
+        check(lib().ovx_server_run_trace(self._h, ctypes.cast(arr, ctypes.c_void_p), len(reqs),
+                                         lat.ctypes.data_as(ctypes.c_void_p), ctypes.byref(mk)))
+        return lat, mk.value
+
+    def close(self):
+        if self._h:
+            lib().ovx_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

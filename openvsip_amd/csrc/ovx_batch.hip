@@ -1,0 +1,342 @@
+// ovx_batch.hip -- one persistent launch over an (instance, rows) tile queue.
+//
+// A batch of uneven fast-convolution jobs (different N and row counts) is cut
+// into tiles of kBatchThreads threads' worth of work: a few whole rows of an
+// LDS-resident job, or a few column/line groups of one row of a four-step
+// job.  A fixed grid of workgroups (one per CU: the LDS footprint is sized for
+// the largest tile) pulls tile indices from an atomic counter until the queue
+// is empty, and runs the same device bodies as the standalone kernels, so a
+// job's result is bitwise the same alone and in a batch.  Four-step jobs need
+// their three passes in order; the host splits a batch into up to three
+// queues (pass 1 + LDS-resident work, pass 2, pass 3) launched back to back
+// in one captured hipGraph.
+#include "ovx_batch.hpp"
+#include "ovx_fs_dev.hpp"
+#include "ovx_hier_dev.hpp"
+#include "ovx_ols_dev.hpp"
+#include "ovx_row_dev.hpp"
+
+namespace ovx {
+
+// LDS of the largest tile (1024 threads): hierarchical rows 17408 float2,
+// four-step groups 4 x 16 x 273.
+constexpr int kBatchLds = 4 * 16 * 273 > 17408 ? 4 * 16 * 273 : 17408;
+
+// The job descriptor of the current tile lives in LDS (copied once per tile
+// by the persistent loop); the tile functions read it from there.
+typedef __attribute__((address_space(3))) const BatchJob lds_job;
+
+template <int N>
+__device__ __noinline__ void tile_hier_fastconv(lds_job &j, int row0, lds_ptr lds)
+{
+  using H = Hier<N>;
+  HierArgs a{};
+  a.x = j.x;
+  a.ldx = j.ldx;
+  a.y = j.y;
+  a.ldy = j.ldy;
+  a.w = j.w;
+  a.ldw = 0;
+  a.rows = j.rows;
+  a.tw_outer = j.tw0;
+  a.tw_wave = j.tw1;
+  a.scale = 1.0f;
+  a.natural = 0;
+  const int lrow = (int)threadIdx.x / H::TR;
+  const int t = (int)threadIdx.x % H::TR;
+  hier_fastconv_body<N>(a, row0, lrow, t, lds + lrow * H::LROW);
+}
+
+template <int N>
+__device__ __noinline__ void tile_hier_fwd(lds_job &j, int row0, lds_ptr lds)
+{
+  using H = Hier<N>;
+  HierArgs a{};
+  a.x = j.x;
+  a.ldx = j.ldx;
+  a.y = j.y;
+  a.ldy = j.ldy;
+  a.rows = j.rows;
+  a.tw_outer = j.tw0;
+  a.tw_wave = j.tw1;
+  a.scale = j.scale;
+  const int lrow = (int)threadIdx.x / H::TR;
+  const int t = (int)threadIdx.x % H::TR;
+  hier_fwd_body<N>(a, row0, lrow, t, lds + lrow * H::LROW);
+}
+
+template <int N, int P>
+__device__ __noinline__ void tile_row_fastconv(lds_job &j, int row0, lds_ptr lds)
+{
+  using E = Engine<N, P, (N / P <= 64)>;
+  const int lrow = (int)threadIdx.x / E::T;
+  const int t = (int)threadIdx.x % E::T;
+  row_fastconv_body<N, P>(j.x, j.ldx, j.w, 0, j.y, j.ldy, j.rows, j.tw0, 1.0f, row0, lrow, t,
+                          lds + lrow * E::LROW);
+}
+
+template <int N, int P>
+__device__ __noinline__ void tile_row_fwd(lds_job &j, int row0, lds_ptr lds)
+{
+  using E = Engine<N, P, (N / P <= 64)>;
+  const int lrow = (int)threadIdx.x / E::T;
+  const int t = (int)threadIdx.x % E::T;
+  row_fft_body<N, P, -1>(j.x, j.ldx, j.y, j.ldy, j.rows, j.tw0, j.scale, row0, lrow, t, lds + lrow * E::LROW);
+}
+
+// Overlap-save segments seg0, seg0 + 1, ... (kBatchThreads / (L/16) of them
+// per tile); a copy past the last segment runs with P = 0, which stores
+// nothing, because ols_body's barriers need every thread.
+template <int L>
+__device__ __noinline__ void tile_ols(lds_job &j, int seg0, lds_ptr lds)
+{
+  OlsArgs a;
+  a.in = j.ols.in;
+  a.N = j.ols.N;
+  a.M = j.ols.M;
+  a.H = j.ols.H;
+  a.tw = j.ols.tw;
+  a.scale = j.ols.scale;
+  a.conj_input = j.ols.conj_input;
+  a.unbiased = j.ols.unbiased;
+  a.support = j.ols.support;
+  a.decimation = j.ols.decimation;
+  a.out_off = j.ols.out_off;
+  a.c_first = j.ols.c_first;
+  a.c_step = j.ols.c_step;
+  a.P = j.ols.P;
+  a.out = j.ols.out;
+  constexpr int T = L / 16;
+  static_assert(T <= kBatchThreads, "segment too long");
+  const int t = (int)threadIdx.x % T;
+  const int copy = (int)threadIdx.x / T;
+  const long seg = (long)seg0 + copy;
+  const long nseg = ((a.P - 1) * a.decimation + a.c_step) / a.c_step; // segments of this job
+  if (seg >= nseg) a.P = 0;
+  ols_body<L, 16>(a, seg, t, lds + copy * Engine<L, 16>::LROW);
+}
+
+int batch_ols_segments_per_tile(int L) { return kBatchThreads / (L / 16); }
+
+__device__ __forceinline__ FsArgs fs_args_of(lds_job &j)
+{
+  FsArgs a{};
+  a.n1 = j.n1;
+  a.n2 = j.n2;
+  a.nbig = j.n;
+  a.tw1 = j.tw0;
+  a.tw2 = j.tw1;
+  a.tlo = j.tlo;
+  a.thi = j.thi;
+  a.sbits = j.sbits;
+  a.wp = j.w;
+  a.ldwp = 0;
+  a.scale = 1.0f;
+  return a;
+}
+
+// pass 1 / 3: column groups; pass 2: line groups
+template <int L, int PASS>
+__device__ __noinline__ void tile_fs(lds_job &j, int row, int group0, lds_ptr lds)
+{
+  using F = FsCfg<L>;
+  constexpr int G = kBatchThreads / F::NT; // groups per tile
+  const int g = (int)threadIdx.x / F::NT;
+  const int tid = (int)threadIdx.x % F::NT;
+  FsArgs a = fs_args_of(j);
+  if (PASS == 1) {
+    a.in = j.x;
+    a.ld_in = j.ldx;
+  } else {
+    a.in = j.y;
+    a.ld_in = j.ldy;
+  }
+  a.out = j.y;
+  a.ld_out = j.ldy;
+  // the host cuts rows into whole tiles: group0 + G never passes the row end
+  const long group = group0 + g;
+  (void)G;
+  if (PASS == 1) fs_col_body<L, F::P, F::C, 0>(a, row, group, tid, lds + g * F::LDS);
+  else if (PASS == 2) fs_row_body<L, F::P, F::C, 0>(a, row, group, tid, lds + g * F::LDS);
+  else fs_col_body<L, F::P, F::C, 1>(a, row, group, tid, lds + g * F::LDS);
+}
+
+// four-step forward Fftm: pass 1 x -> ws (column FFT + twiddle), pass 2
+// ws -> y (row FFT, LDS-staged natural-order store, scale)
+template <int L, int PASS>
+__device__ __noinline__ void tile_fs_fft(lds_job &j, int row, int group0, lds_ptr lds)
+{
+  using F = FsCfg<L>;
+  const int g = (int)threadIdx.x / F::NT;
+  const int tid = (int)threadIdx.x % F::NT;
+  FsArgs a = fs_args_of(j);
+  a.scale = j.scale;
+  const long group = group0 + g;
+  if (PASS == 1) {
+    a.in = j.x;
+    a.ld_in = j.ldx;
+    a.out = j.ws;
+    a.ld_out = j.n;
+    fs_col_body<L, F::P, F::C, 0>(a, row, group, tid, lds + g * F::LDS);
+  } else {
+    a.in = j.ws;
+    a.ld_in = j.n;
+    a.out = j.y;
+    a.ld_out = j.ldy;
+    fs_row_body<L, F::P, F::C, 1>(a, row, group, tid, lds + g * F::LDS);
+  }
+}
+
+template <int PASS>
+__device__ __forceinline__ void tile_fs_fft_dispatch(lds_job &j, int row, int group0, lds_ptr lds)
+{
+  const int L = PASS == 2 ? j.n2 : j.n1;
+  switch (L) {
+  case 256: tile_fs_fft<256, PASS>(j, row, group0, lds); break;
+  case 512: tile_fs_fft<512, PASS>(j, row, group0, lds); break;
+  default: break;
+  }
+}
+
+template <int PASS>
+__device__ __forceinline__ void tile_fs_dispatch(lds_job &j, int row, int group0, lds_ptr lds)
+{
+  const int L = PASS == 2 ? j.n2 : j.n1;
+  switch (L) {
+  case 256: tile_fs<256, PASS>(j, row, group0, lds); break;
+  case 512: tile_fs<512, PASS>(j, row, group0, lds); break;
+  default: break;
+  }
+}
+
+__global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restrict__ jobs,
+                                                          const BatchTile *__restrict__ tiles,
+                                                          const int *__restrict__ ntiles, int *counter)
+{
+  __shared__ float2 lds_mem[kBatchLds];
+  __shared__ BatchJob s_job;
+  __shared__ int s_tile;
+  const lds_ptr lds = as_lds(lds_mem);
+  lds_job &j = *(lds_job *)&s_job;
+  static_assert(sizeof(BatchJob) % 4 == 0 && sizeof(BatchJob) / 4 <= kBatchThreads, "job copy");
+  const int total = *ntiles;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    // uniform across the workgroup: keep it (and the job loaded through it)
+    // in scalar registers
+    const int ti = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads(); // every thread has read s_tile before it is overwritten
+    if (ti >= total) break;
+    const BatchTile tile = tiles[ti];
+    if (threadIdx.x < sizeof(BatchJob) / 4)
+      ((int *)&s_job)[threadIdx.x] = ((const int *)&jobs[tile.job])[threadIdx.x];
+    __syncthreads();
+    switch (tile.op) {
+    case BT_HIER_FASTCONV:
+      switch (j.n) {
+      case 1024: tile_hier_fastconv<1024>(j, tile.a, lds); break;
+      case 2048: tile_hier_fastconv<2048>(j, tile.a, lds); break;
+      case 4096: tile_hier_fastconv<4096>(j, tile.a, lds); break;
+      case 8192: tile_hier_fastconv<8192>(j, tile.a, lds); break;
+      case 16384: tile_hier_fastconv<16384>(j, tile.a, lds); break;
+      default: break;
+      }
+      break;
+    case BT_HIER_FFT_FWD:
+      switch (j.n) {
+      case 1024: tile_hier_fwd<1024>(j, tile.a, lds); break;
+      case 2048: tile_hier_fwd<2048>(j, tile.a, lds); break;
+      case 4096: tile_hier_fwd<4096>(j, tile.a, lds); break;
+      case 8192: tile_hier_fwd<8192>(j, tile.a, lds); break;
+      case 16384: tile_hier_fwd<16384>(j, tile.a, lds); break;
+      default: break;
+      }
+      break;
+    case BT_ROW_FASTCONV:
+      switch (j.n) {
+      case 256: tile_row_fastconv<256, 16>(j, tile.a, lds); break;
+      case 512: tile_row_fastconv<512, 16>(j, tile.a, lds); break;
+      case 3072: tile_row_fastconv<3072, 12>(j, tile.a, lds); break;
+      case 6144: tile_row_fastconv<6144, 12>(j, tile.a, lds); break;
+      case 12288: tile_row_fastconv<12288, 12>(j, tile.a, lds); break;
+      default: break;
+      }
+      break;
+    case BT_ROW_FFT_FWD:
+      switch (j.n) {
+      case 256: tile_row_fwd<256, 16>(j, tile.a, lds); break;
+      case 512: tile_row_fwd<512, 16>(j, tile.a, lds); break;
+      case 3072: tile_row_fwd<3072, 12>(j, tile.a, lds); break;
+      case 6144: tile_row_fwd<6144, 12>(j, tile.a, lds); break;
+      case 12288: tile_row_fwd<12288, 12>(j, tile.a, lds); break;
+      default: break;
+      }
+      break;
+    case BT_FS_FFT_A: tile_fs_fft_dispatch<1>(j, tile.a, tile.b, lds); break;
+    case BT_FS_FFT_B: tile_fs_fft_dispatch<2>(j, tile.a, tile.b, lds); break;
+    case BT_OLS:
+      switch (j.ols_len) {
+      case 1024: tile_ols<1024>(j, tile.a, lds); break;
+      case 2048: tile_ols<2048>(j, tile.a, lds); break;
+      case 4096: tile_ols<4096>(j, tile.a, lds); break;
+      case 8192: tile_ols<8192>(j, tile.a, lds); break;
+      case 16384: tile_ols<16384>(j, tile.a, lds); break;
+      default: break;
+      }
+      break;
+    case BT_FS_A: tile_fs_dispatch<1>(j, tile.a, tile.b, lds); break;
+    case BT_FS_B: tile_fs_dispatch<2>(j, tile.a, tile.b, lds); break;
+    case BT_FS_C: tile_fs_dispatch<3>(j, tile.a, tile.b, lds); break;
+    default: break;
+    }
+    __syncthreads(); // LDS reuse by the next tile
+  }
+}
+
+__global__ void k_reset_counter(int *counter) { *counter = 0; }
+
+hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int *counter, int grid,
+                        hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBatchThreads), 0, s, jobs, tiles, ntiles, counter);
+  return hipGetLastError();
+}
+
+bool batch_supported(int path, int n)
+{
+  if (path == BP_HIER) return hier_supported(n);
+  if (path == BP_ROW) return n == 256 || n == 512 || n == 3072 || n == 6144 || n == 12288;
+  if (path == BP_FS) return n == 65536 || n == 131072 || n == 262144;
+  return false;
+}
+
+// Batch tiles of the 3 * 2^k lengths use 12 values per thread, so that 1024
+// threads fit the 128-VGPR budget of a 1024-thread workgroup.
+int batch_row_p(int n) { return (n == 256 || n == 512) ? 16 : 12; }
+
+int batch_rows_per_tile(int path, int n)
+{
+  if (path == BP_HIER) return kBatchThreads / (64 * (n / 1024));
+  if (path == BP_ROW) return kBatchThreads / (n / batch_row_p(n));
+  return 1;
+}
+
+int batch_groups_per_tile(int n_line)
+{
+  if (n_line == 256) return kBatchThreads / FsCfg<256>::NT;
+  if (n_line == 512) return kBatchThreads / FsCfg<512>::NT;
+  return 0;
+}
+
+int batch_fs_lines_per_group(int n_line)
+{
+  if (n_line == 256) return FsCfg<256>::C;
+  if (n_line == 512) return FsCfg<512>::C;
+  return 0;
+}
+
+} // namespace ovx

@@ -1,0 +1,66 @@
+// ovx_batch.hpp -- persistent tile-queue kernel for batches of uneven jobs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ovx_ols.hpp"
+
+namespace ovx {
+
+// Kernel path of one job (which device body its tiles run).
+enum BatchPath { BP_HIER = 1, BP_ROW = 2, BP_FS = 3 };
+// Tile operations.
+enum BatchOp {
+  BT_HIER_FASTCONV = 1, // rows [a, a + rows_per_tile) of a hierarchical-path job
+  BT_ROW_FASTCONV = 2,  // rows [a, ...) of a register-Stockham-path job
+  BT_FS_A = 3,          // four-step fastconv pass 1: row a, column groups b, b+1, ...
+  BT_FS_B = 4,          // pass 2: row a, line groups b, ...
+  BT_FS_C = 5,          // pass 3: row a, column groups b, ...
+  BT_HIER_FFT_FWD = 6,  // forward Fftm rows of a hierarchical-path job
+  BT_ROW_FFT_FWD = 7,   // forward Fftm rows of a register-Stockham-path job
+  BT_FS_FFT_A = 8,      // four-step forward Fftm pass 1: x -> workspace
+  BT_FS_FFT_B = 9,      // pass 2: workspace -> y (natural order)
+  BT_OLS = 10           // one overlap-save segment (a = segment index)
+};
+
+struct BatchJob {
+  int path;
+  int n;
+  long rows;
+  const float2 *x;
+  long ldx;
+  float2 *y;
+  long ldy;
+  const float2 *w;   // prepared spectrum (path layout, 1/n folded in)
+  const float2 *tw0; // HIER: outer table; ROW: engine table; FS: n1 line table
+  const float2 *tw1; // HIER: 1024-point wave table; FS: n2 line table
+  const float2 *tlo, *thi;
+  int sbits;
+  int n1, n2;
+  float scale;       // forward Fftm scale
+  float2 *ws;        // four-step Fftm workspace (rows x n)
+  OlsArgs ols;       // overlap-save segment parameters
+  int ols_len;       // segment length L
+};
+
+struct BatchTile {
+  int job;
+  int op;
+  int a; // first row
+  int b; // first group (four-step)
+};
+
+// Threads per persistent workgroup and tile geometry.
+constexpr int kBatchThreads = 1024;
+int batch_rows_per_tile(int path, int n);
+int batch_row_p(int n); // values per thread of a register-Stockham batch tile
+int batch_ols_segments_per_tile(int L);
+int batch_groups_per_tile(int n_line);
+int batch_fs_lines_per_group(int n_line); // C of a four-step group of lines of length n_line
+bool batch_supported(int path, int n);
+
+// One persistent launch over tiles[0 .. *ntiles): `grid` workgroups pull
+// tiles with an atomic counter (which this launch resets first).
+hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int *counter, int grid,
+                        hipStream_t s);
+
+} // namespace ovx

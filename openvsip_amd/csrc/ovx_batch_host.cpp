@@ -1,0 +1,330 @@
+// ovx_batch_host.cpp -- batches of uneven fastconv jobs (C-ABI part):
+// tile-queue construction, device tables, persistent launches captured once
+// into a hipGraph and replayed, and the plain one-launch-per-job path it is
+// compared with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "ovx_batch.hpp"
+#include "ovx_batch_host.hpp"
+#include "ovx_plan.hpp"
+
+namespace ovx {
+
+// ---------------------------------------------------------------- shared paths
+// One default kernel path (twiddle tables) per transform length, shared by
+// batches, replicas and the server.
+static std::mutex g_path_mu;
+static std::map<long, std::unique_ptr<LineFFT>> g_paths;
+
+// Variant the batch tiles of length n run: the 3 * 2^k lengths use the
+// 12-values-per-thread register-Stockham configuration (variant 1), which
+// fits the persistent kernel's 1024-thread register budget.
+int batch_variant(long n) { return (n == 3072 || n == 6144 || n == 12288) ? 1 : 0; }
+
+const LineFFT *shared_path(long n, int *status)
+{
+  std::lock_guard<std::mutex> g(g_path_mu);
+  auto it = g_paths.find(n);
+  if (it != g_paths.end()) {
+    *status = OVX_OK;
+    return it->second.get();
+  }
+  std::unique_ptr<LineFFT> f(new (std::nothrow) LineFFT());
+  if (!f) {
+    *status = fail(OVX_ERR_NOMEM, "host allocation");
+    return nullptr;
+  }
+  *status = f->init(n, batch_variant(n));
+  if (*status) return nullptr;
+  const LineFFT *p = f.get();
+  g_paths[n] = std::move(f);
+  return p;
+}
+
+int batch_path_of(const LineFFT &f)
+{
+  if (f.kind == LineFFT::HIER) return BP_HIER;
+  if (f.kind == LineFFT::LDS) return BP_ROW;
+  if (f.kind == LineFFT::FOURSTEP) return BP_FS;
+  return 0;
+}
+
+int device_cu_count()
+{
+  static int cus = 0;
+  if (!cus) {
+    hipDeviceProp_t p;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return 256;
+    cus = p.multiProcessorCount;
+  }
+  return cus;
+}
+
+// Fill `job` for a fastconv of n x rows against a prepared spectrum.
+int make_batch_job(const LineFFT &f, long rows, const float2 *x, long ldx, float2 *y, long ldy, const float2 *wprep,
+                   BatchJob *job)
+{
+  BatchJob j{};
+  j.path = batch_path_of(f);
+  j.n = (int)f.n;
+  if (!batch_supported(j.path, j.n))
+    return fail(OVX_ERR_UNSUPPORTED, "length %ld has no batch tile (path %d)", f.n, j.path);
+  if (!f.tables_ready()) return fail(OVX_ERR_STATE, "length %ld: twiddle tables missing", f.n);
+  if (rows > 0x7fffffff) return fail(OVX_ERR_UNSUPPORTED, "too many rows in one job");
+  j.rows = rows;
+  j.x = x;
+  j.ldx = ldx;
+  j.y = y;
+  j.ldy = ldy;
+  j.w = wprep;
+  j.scale = 1.0f;
+  if (j.path == BP_HIER) {
+    j.tw0 = (const float2 *)f.tw_outer.p;
+    j.tw1 = (const float2 *)f.tw_wave.p;
+  } else if (j.path == BP_ROW) {
+    // the tile body is Engine<n, batch_row_p(n)>: the path must be exactly
+    // that configuration, or its twiddle table would not match
+    if (f.rc.p != batch_row_p(j.n))
+      return fail(OVX_ERR_STATE, "length %ld: path has %d values per thread, batch tile needs %d", f.n, f.rc.p,
+                  batch_row_p(j.n));
+    j.tw0 = (const float2 *)f.tw.p;
+  } else {
+    j.tw0 = (const float2 *)f.tw1.p;
+    j.tw1 = (const float2 *)f.tw2.p;
+    j.tlo = (const float2 *)f.tlo.p;
+    j.thi = (const float2 *)f.thi.p;
+    j.sbits = f.sbits;
+    j.n1 = (int)f.n1;
+    j.n2 = (int)f.n2;
+  }
+  *job = j;
+  return OVX_OK;
+}
+
+// Tiles of job `ji`: LDS-resident rows go to `lds`, the three four-step
+// passes to pass[0..2].
+void make_batch_tiles(const BatchJob &j, int ji, std::vector<BatchTile> &lds, std::vector<BatchTile> pass[3])
+{
+  if (j.path == BP_HIER || j.path == BP_ROW) {
+    int rpt = batch_rows_per_tile(j.path, j.n);
+    int op = j.path == BP_HIER ? BT_HIER_FASTCONV : BT_ROW_FASTCONV;
+    for (long r = 0; r < j.rows; r += rpt) lds.push_back(BatchTile{ ji, op, (int)r, 0 });
+    return;
+  }
+  const int gcol = j.n2 / batch_fs_lines_per_group(j.n1), tcol = batch_groups_per_tile(j.n1);
+  const int grow = j.n1 / batch_fs_lines_per_group(j.n2), trow = batch_groups_per_tile(j.n2);
+  for (long r = 0; r < j.rows; ++r) {
+    for (int g = 0; g < gcol; g += tcol) pass[0].push_back(BatchTile{ ji, BT_FS_A, (int)r, g });
+    for (int g = 0; g < grow; g += trow) pass[1].push_back(BatchTile{ ji, BT_FS_B, (int)r, g });
+    for (int g = 0; g < gcol; g += tcol) pass[2].push_back(BatchTile{ ji, BT_FS_C, (int)r, g });
+  }
+}
+
+// Spread LDS-resident tiles over the launches so that each launch has about
+// the same number of tiles (every tile is ~16K points of work).
+void balance_queues(std::vector<BatchTile> &lds, std::vector<BatchTile> pass[3], std::vector<BatchTile> q[3])
+{
+  for (int k = 0; k < 3; ++k) q[k] = pass[k];
+  size_t total = lds.size() + pass[0].size() + pass[1].size() + pass[2].size();
+  int nq = pass[0].empty() ? 1 : 3;
+  size_t target = (total + nq - 1) / nq;
+  size_t li = 0;
+  for (int k = 0; k < nq; ++k) {
+    while (li < lds.size() && (q[k].size() < target || k == nq - 1)) q[k].push_back(lds[li++]);
+  }
+}
+
+GraphBatch::~GraphBatch() { release_graph(); }
+
+void GraphBatch::release_graph()
+{
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+}
+
+// Upload jobs and up to three tile queues; capture the launches once.
+int GraphBatch::build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> q[3], hipStream_t s)
+{
+  release_graph();
+  size_t nt = q[0].size() + q[1].size() + q[2].size();
+  int st;
+  if ((st = d_jobs.alloc(std::max<size_t>(jobs.size(), 1) * sizeof(BatchJob))) ||
+      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc(8 * sizeof(int))))
+    return st;
+  std::vector<BatchTile> all;
+  all.reserve(nt);
+  int counts[8] = { 0 };
+  nq = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (q[k].empty()) continue;
+    offset[nq] = (int)all.size();
+    counts[nq] = (int)q[k].size();
+    all.insert(all.end(), q[k].begin(), q[k].end());
+    ++nq;
+  }
+  hipError_t e = hipSuccess;
+  if (!jobs.empty()) e = hipMemcpyAsync(d_jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && !all.empty())
+    e = hipMemcpyAsync(d_tiles.p, all.data(), all.size() * sizeof(BatchTile), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_meta.p, counts, sizeof(counts), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "batch table upload");
+  ntiles = nt;
+  if (nq == 0) return OVX_OK;
+
+  hipStream_t cap;
+  e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "capture stream");
+  e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    int *meta = (int *)d_meta.p;
+    const BatchTile *tiles = (const BatchTile *)d_tiles.p;
+    for (int k = 0; k < nq && e == hipSuccess; ++k)
+      e = launch_batch((const BatchJob *)d_jobs.p, tiles + offset[k], meta + k, meta + 4 + k, device_cu_count(),
+                       cap);
+    hipError_t e2 = hipStreamEndCapture(cap, &graph);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cap);
+  if (e != hipSuccess) {
+    release_graph();
+    return hip_fail(e, "batch graph capture");
+  }
+  return OVX_OK;
+}
+
+int GraphBatch::launch(hipStream_t s) const
+{
+  if (nq == 0) return OVX_OK;
+  hipError_t e = hipGraphLaunch(exec, s);
+  if (e != hipSuccess) return hip_fail(e, "batch graph launch");
+  return OVX_OK;
+}
+
+} // namespace ovx
+
+using namespace ovx;
+
+// ---------------------------------------------------------------- C-ABI
+struct ovx_batch_s {
+  struct HostJob {
+    long n, rows;
+    const void *x;
+    long ldx;
+    int replica;
+    void *y;
+    long ldy;
+  };
+  std::vector<HostJob> jobs;
+  std::vector<BatchJob> djobs;
+  std::vector<const LineFFT *> paths;
+  std::vector<const float2 *> preps;
+  GraphBatch gb;
+  bool committed = false;
+};
+
+extern "C" int ovx_batch_create(ovx_batch *b)
+{
+  if (!b) return fail(OVX_ERR_INVALID, "null batch pointer");
+  *b = new (std::nothrow) ovx_batch_s();
+  if (!*b) return fail(OVX_ERR_NOMEM, "host allocation");
+  return OVX_OK;
+}
+
+extern "C" int ovx_batch_destroy(ovx_batch b)
+{
+  if (!b) return fail(OVX_ERR_INVALID, "null batch");
+  delete b;
+  return OVX_OK;
+}
+
+extern "C" int ovx_batch_add_fastconv(ovx_batch b, long n, long rows, const void *x, long ldx, int replica, void *y,
+                                      long ldy)
+{
+  if (!b) return fail(OVX_ERR_INVALID, "null batch");
+  if (b->committed) return fail(OVX_ERR_STATE, "batch already committed");
+  if (!x || !y) return fail(OVX_ERR_INVALID, "null data pointer");
+  if (n < 1 || rows < 0) return fail(OVX_ERR_INVALID, "bad job size");
+  if (rows > 1 && (ldx < n || ldy < n)) return fail(OVX_ERR_INVALID, "row stride shorter than n");
+  b->jobs.push_back(ovx_batch_s::HostJob{ n, rows, x, ldx, replica, y, ldy });
+  return OVX_OK;
+}
+
+extern "C" int ovx_batch_commit(ovx_batch b, void *stream)
+{
+  if (!b) return fail(OVX_ERR_INVALID, "null batch");
+  if (b->committed) return fail(OVX_ERR_STATE, "batch already committed");
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<BatchTile> lds, pass[3], q[3];
+  b->djobs.clear();
+  b->paths.clear();
+  b->preps.clear();
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    const auto &h = b->jobs[i];
+    int st;
+    const LineFFT *f = shared_path(h.n, &st);
+    if (!f) return st;
+    const float2 *prep = nullptr;
+    long rn = 0;
+    if ((st = replica_prepared(h.replica, &prep, &rn, f))) return st;
+    if (rn != h.n) return fail(OVX_ERR_INVALID, "replica %d has length %ld, job %zu has %ld", h.replica, rn, i, h.n);
+    BatchJob j;
+    if ((st = make_batch_job(*f, h.rows, (const float2 *)h.x, h.ldx, (float2 *)h.y, h.ldy, prep, &j))) return st;
+    b->djobs.push_back(j);
+    b->paths.push_back(f);
+    b->preps.push_back(prep);
+  }
+  // largest jobs first inside each queue
+  std::vector<int> order(b->djobs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->djobs[a].n > b->djobs[c].n; });
+  for (int i : order) make_batch_tiles(b->djobs[i], i, lds, pass);
+  balance_queues(lds, pass, q);
+  int st = b->gb.build(b->djobs, q, s);
+  if (st) return st;
+  b->committed = true;
+  return OVX_OK;
+}
+
+extern "C" int ovx_batch_exec(ovx_batch b, void *stream)
+{
+  if (!b) return fail(OVX_ERR_INVALID, "null batch");
+  if (!b->committed) return fail(OVX_ERR_STATE, "batch not committed");
+  return b->gb.launch((hipStream_t)stream);
+}
+
+extern "C" int ovx_batch_exec_plain(ovx_batch b, void *stream)
+{
+  if (!b) return fail(OVX_ERR_INVALID, "null batch");
+  if (!b->committed) return fail(OVX_ERR_STATE, "batch not committed");
+  hipStream_t s = (hipStream_t)stream;
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    const auto &h = b->jobs[i];
+    int st = b->paths[i]->fastconv_prepared((const float2 *)h.x, h.ldx, b->preps[i], (float2 *)h.y, h.ldy, h.rows, s);
+    if (st) return st;
+  }
+  return OVX_OK;
+}
+
+extern "C" long ovx_batch_tiles(ovx_batch b)
+{
+  if (!b) return -1;
+  return (long)b->gb.ntiles;
+}
+
+extern "C" int ovx_batch_launches(ovx_batch b)
+{
+  if (!b) return -1;
+  return b->gb.nq;
+}

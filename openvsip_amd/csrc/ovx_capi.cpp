@@ -17,6 +17,7 @@
 
 #include "../../include/ovxhip.h"
 #include "ovx_fourstep.hpp"
+#include "ovx_hier.hpp"
 #include "ovx_internal.hpp"
 #include "ovx_plan.hpp"
 
@@ -85,12 +86,30 @@ static int make_row_twiddles(const RowConfig &rc, DevBuf &tw)
   return OVX_OK;
 }
 
-int LineFFT::init(long n_, int variant)
+int LineFFT::variants(long n)
+{
+  if (n >= 1 && n <= kMaxRowN) return (hier_supported(n) ? 1 : 0) + row_variants((int)n);
+  return 1;
+}
+
+int LineFFT::init(long n_, int variant_)
 {
   n = n_;
+  variant = variant_;
+  if (variant < 0 || variant >= variants(n)) variant = 0;
+  if (hier_supported(n) && variant == 0) {
+    int st;
+    long outer = std::max<long>((n / 1024 - 1) * 1024, 1);
+    if ((st = tw_outer.alloc(outer * 8)) || (st = tw_wave.alloc((size_t)hier_wave_twiddle_count() * 8))) return st;
+    hipError_t e = launch_hier_twiddles(n, (float2 *)tw_outer.p, (float2 *)tw_wave.p, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hip_fail(e, "twiddle kernels");
+    kind = HIER;
+    return OVX_OK;
+  }
   if (n >= 1 && n <= kMaxRowN && row_variants((int)n) > 0) {
-    if (!row_config((int)n, variant < row_variants((int)n) ? variant : 0, &rc))
-      return fail(OVX_ERR_UNSUPPORTED, "no LDS-resident configuration for n=%ld", n);
+    int v = hier_supported(n) ? variant - 1 : variant;
+    if (!row_config((int)n, v, &rc)) return fail(OVX_ERR_UNSUPPORTED, "no LDS-resident configuration for n=%ld", n);
     kind = LDS;
     return make_row_twiddles(rc, tw);
   }
@@ -138,6 +157,7 @@ const char *LineFFT::name() const
 {
   switch (kind) {
   case LDS: return "lds-resident";
+  case HIER: return "hierarchical-lds";
   case FOURSTEP: return "four-step";
   default: return "none";
   }
@@ -145,13 +165,53 @@ const char *LineFFT::name() const
 
 size_t LineFFT::device_bytes() const
 {
-  return tw.bytes + tw1.bytes + tw2.bytes + tlo.bytes + thi.bytes + work.bytes + wp.bytes;
+  return tw.bytes + tw1.bytes + tw2.bytes + tlo.bytes + thi.bytes + work.bytes + wp.bytes + tw_outer.bytes +
+         tw_wave.bytes;
+}
+
+bool LineFFT::tables_ready() const
+{
+  switch (kind) {
+  case LDS: return tw.p != nullptr;
+  case HIER: return tw_outer.p != nullptr && tw_wave.p != nullptr;
+  case FOURSTEP: return tw1.p && tw2.p && tlo.p && thi.p;
+  default: return false;
+  }
+}
+
+static HierArgs hier_args(const LineFFT &f)
+{
+  HierArgs a{};
+  a.tw_outer = (const float2 *)f.tw_outer.p;
+  a.tw_wave = (const float2 *)f.tw_wave.p;
+  a.scale = 1.0f;
+  return a;
+}
+
+static bool offsets_fit(long rows_per_block, long ld)
+{
+  return (unsigned long)rows_per_block * (unsigned long)std::max(ld, 0l) * 8ul < (1ul << 32);
 }
 
 int LineFFT::transform(int dir, const float2 *in, long ld_in, float2 *out, long ld_out, long lines, float scale,
                        hipStream_t s) const
 {
   if (lines <= 0) return OVX_OK;
+  if (!tables_ready()) return fail(OVX_ERR_STATE, "transform plan has no twiddle tables");
+  if (kind == HIER) {
+    if (!offsets_fit(hier_rows_per_block(n), std::max(ld_in, ld_out)))
+      return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
+    HierArgs a = hier_args(*this);
+    a.x = in;
+    a.ldx = ld_in;
+    a.y = out;
+    a.ldy = ld_out;
+    a.rows = lines;
+    a.scale = scale;
+    hipError_t e = launch_hier(dir < 0 ? HIER_FWD : HIER_INV, n, a, s);
+    if (e != hipSuccess) return hip_fail(e, "row fft launch");
+    return OVX_OK;
+  }
   if (kind == LDS) {
     if ((unsigned long)rc.rows_per_block * (unsigned long)std::max(ld_in, ld_out) * 8ul >= (1ul << 32))
       return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
@@ -186,6 +246,37 @@ int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, floa
                       float scale, hipStream_t s) const
 {
   if (lines <= 0) return OVX_OK;
+  if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
+  if (kind == HIER) {
+    if (!offsets_fit(hier_rows_per_block(n), std::max(std::max(ldx, ldy), ldw)))
+      return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
+    HierArgs a = hier_args(*this);
+    a.x = x;
+    a.ldx = ldx;
+    a.y = y;
+    a.ldy = ldy;
+    a.rows = lines;
+    if (ldw == 0) {
+      // one spectrum: permute it once into the kernel's [k1][k2] order with
+      // the scale folded in
+      int st = wp.alloc((size_t)n * 8);
+      if (st) return st;
+      hipError_t e = launch_hier_permute(w, 0, (float2 *)wp.p, n, 1, scale, s);
+      if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
+      a.w = (const float2 *)wp.p;
+      a.ldw = 0;
+      a.natural = 0;
+      a.scale = 1.0f;
+    } else {
+      a.w = w;
+      a.ldw = ldw;
+      a.natural = 1;
+      a.scale = scale;
+    }
+    hipError_t e = launch_hier(HIER_FASTCONV, n, a, s);
+    if (e != hipSuccess) return hip_fail(e, "fastconv launch");
+    return OVX_OK;
+  }
   if (kind == LDS) {
     long ld = std::max(std::max(ldx, ldy), ldw);
     if ((unsigned long)rc.rows_per_block * (unsigned long)ld * 8ul >= (1ul << 32))
@@ -216,6 +307,58 @@ int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, floa
     return OVX_OK;
   }
   return fail(OVX_ERR_UNSUPPORTED, "no fastconv path");
+}
+
+int LineFFT::prepare_spectrum(const float2 *w, float2 *wp_out, float scale, hipStream_t s) const
+{
+  hipError_t e;
+  if (kind == HIER) e = launch_hier_permute(w, 0, wp_out, n, 1, scale, s);
+  else if (kind == FOURSTEP) e = launch_fs_permute(w, 0, wp_out, n1, n2, 1, scale, s);
+  else e = launch_scale_copy(w, wp_out, n, scale, s);
+  if (e != hipSuccess) return hip_fail(e, "spectrum preparation");
+  return OVX_OK;
+}
+
+int LineFFT::fastconv_prepared(const float2 *x, long ldx, const float2 *wprep, float2 *y, long ldy, long lines,
+                               hipStream_t s) const
+{
+  if (lines <= 0) return OVX_OK;
+  if (!tables_ready() || !wprep) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables or spectrum");
+  if (kind == HIER) {
+    if (!offsets_fit(hier_rows_per_block(n), std::max(ldx, ldy)))
+      return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
+    HierArgs a = hier_args(*this);
+    a.x = x;
+    a.ldx = ldx;
+    a.y = y;
+    a.ldy = ldy;
+    a.rows = lines;
+    a.w = wprep;
+    a.ldw = 0;
+    a.natural = 0;
+    a.scale = 1.0f;
+    hipError_t e = launch_hier(HIER_FASTCONV, n, a, s);
+    if (e != hipSuccess) return hip_fail(e, "fastconv launch");
+    return OVX_OK;
+  }
+  if (kind == FOURSTEP) {
+    if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step fastconv");
+    FsArgs a = fs_args(*this);
+    a.wp = wprep;
+    a.ldwp = 0;
+    a.in = x;
+    a.ld_in = ldx;
+    a.out = y;
+    a.ld_out = ldy;
+    hipError_t e;
+    if ((e = launch_fs_col(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
+    a.in = y;
+    a.ld_in = ldy;
+    if ((e = launch_fs_row(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step row pass");
+    if ((e = launch_fs_col(1, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
+    return OVX_OK;
+  }
+  return fastconv(x, ldx, wprep, 0, y, ldy, lines, 1.0f, s);
 }
 
 } // namespace ovx
@@ -268,6 +411,18 @@ extern "C" int ovx_init(int device)
 extern "C" const char *ovx_last_error(void) { return g_err.c_str(); }
 extern "C" const char *ovx_version(void) { return "ovxhip 0.1 (gfx950)"; }
 extern "C" long ovx_max_length(void) { return 2048l * 2048l; }
+extern "C" int ovx_length_variants(long n)
+{
+  if (!ovx_supported_length(n)) return 0;
+  return LineFFT::variants(n);
+}
+
+static int variant_of(unsigned flags)
+{
+  int v = (int)((flags >> 8) & 0xffu);
+  return v ? v - 1 : 0;
+}
+
 extern "C" int ovx_supported_length(long n)
 {
   if (n >= 1 && n <= kMaxRowN) return row_variants((int)n) > 0 ? 1 : 0;
@@ -295,7 +450,7 @@ extern "C" int ovx_fftm_plan_create(ovx_plan *plan, long rows, long cols, int ax
   p->flags = flags;
   long n = axis == OVX_ROW ? cols : rows;
   if (n > 0) {
-    int st = p->line.init(n, 0);
+    int st = p->line.init(n, variant_of(flags));
     if (st) return st;
   }
   *plan = p.release();
@@ -385,7 +540,7 @@ extern "C" int ovx_fastconv_plan_create(ovx_plan *plan, long rows, long n, float
   p->cols = n;
   p->scale = scale;
   p->flags = flags;
-  int st = p->line.init(n, 0);
+  int st = p->line.init(n, variant_of(flags));
   if (st) return st;
   *plan = p.release();
   return OVX_OK;
@@ -420,10 +575,14 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
 }
 
 // ---------------------------------------------------------------- replicas
+// A replica spectrum kept resident on the device, in natural order and, for
+// the default kernel path of its length, prepared once in the layout that
+// path's fastconv kernel reads (permuted, scaled by 1/n).
 namespace {
 struct Replica {
   long n = 0;
-  DevBuf spec;
+  DevBuf spec, prep;
+  int prep_kind = 0, prep_variant = -1;
 };
 std::mutex g_rep_mu;
 std::map<int, std::unique_ptr<Replica>> g_replicas;
@@ -439,6 +598,8 @@ extern "C" int ovx_replica_create(long n, const void *replica, long m, int is_sp
   r->n = n;
   int st = r->spec.alloc((size_t)n * 8);
   if (st) return st;
+  LineFFT f;
+  if ((st = f.init(n, 0))) return st;
   if (is_spectrum) {
     hipError_t e = hipMemcpy(r->spec.p, replica, (size_t)n * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "H2D copy");
@@ -449,12 +610,16 @@ extern "C" int ovx_replica_create(long n, const void *replica, long m, int is_sp
     hipError_t e = hipMemset(tmp.p, 0, (size_t)n * 8);
     if (e == hipSuccess && m > 0) e = hipMemcpy(tmp.p, replica, (size_t)m * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "replica upload");
-    LineFFT f;
-    if ((st = f.init(n, 0))) return st;
     if ((st = f.transform(-1, (const float2 *)tmp.p, n, (float2 *)r->spec.p, n, 1, 1.0f, nullptr))) return st;
     e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return hip_fail(e, "replica transform");
   }
+  if ((st = r->prep.alloc((size_t)n * 8))) return st;
+  if ((st = f.prepare_spectrum((const float2 *)r->spec.p, (float2 *)r->prep.p, 1.0f / (float)n, nullptr))) return st;
+  hipError_t e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) return hip_fail(e, "replica preparation");
+  r->prep_kind = f.kind;
+  r->prep_variant = f.variant;
   std::lock_guard<std::mutex> g(g_rep_mu);
   *id = g_next_rep++;
   g_replicas[*id] = std::move(r);
@@ -479,6 +644,53 @@ extern "C" int ovx_replica_spectrum(int id, const void **spectrum, long *n)
   *spectrum = it->second->spec.p;
   if (n) *n = it->second->n;
   return OVX_OK;
+}
+
+namespace ovx {
+int replica_prepared(int id, const float2 **prep, long *n, const LineFFT *f)
+{
+  std::lock_guard<std::mutex> g(g_rep_mu);
+  auto it = g_replicas.find(id);
+  if (it == g_replicas.end()) return fail(OVX_ERR_INVALID, "unknown replica id %d", id);
+  // register-Stockham paths read the spectrum in natural order whatever
+  // their variant; the other paths need the layout of their own variant
+  bool ok = it->second->prep_kind == f->kind &&
+            (f->kind == LineFFT::LDS || it->second->prep_variant == f->variant);
+  if (!ok) return fail(OVX_ERR_STATE, "replica %d is not prepared for this kernel path", id);
+  *prep = (const float2 *)it->second->prep.p;
+  *n = it->second->n;
+  return OVX_OK;
+}
+} // namespace ovx
+
+extern "C" int ovx_fastconv_exec_replica(ovx_plan p, const void *x, long ldx, int replica, void *y, long ldy,
+                                         long rows, void *stream)
+{
+  int st = check_plan(p, PLAN_FASTCONV);
+  if (st) return st;
+  if (!x || !y) return fail(OVX_ERR_INVALID, "null data pointer");
+  if (p->flags & OVX_FLAG_HOST_POINTERS)
+    return fail(OVX_ERR_UNSUPPORTED, "resident-replica fastconv takes device pointers");
+  if (rows < 0 || rows > p->rows) return fail(OVX_ERR_INVALID, "rows %ld outside the plan's %ld", rows, p->rows);
+  long n = p->cols;
+  if (rows > 1 && (ldx < n || ldy < n)) return fail(OVX_ERR_INVALID, "row stride shorter than n");
+  const float2 *spec, *prep;
+  int pk, pv;
+  {
+    std::lock_guard<std::mutex> g(g_rep_mu);
+    auto it = g_replicas.find(replica);
+    if (it == g_replicas.end()) return fail(OVX_ERR_INVALID, "unknown replica id %d", replica);
+    if (it->second->n != n) return fail(OVX_ERR_INVALID, "replica length %ld != plan length %ld", it->second->n, n);
+    spec = (const float2 *)it->second->spec.p;
+    prep = (const float2 *)it->second->prep.p;
+    pk = it->second->prep_kind;
+    pv = it->second->prep_variant;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (rows == 0) return OVX_OK;
+  if (pk == p->line.kind && (pk == LineFFT::LDS || pv == p->line.variant) && p->scale == 1.0f / (float)n)
+    return p->line.fastconv_prepared((const float2 *)x, ldx, prep, (float2 *)y, ldy, rows, s);
+  return p->line.fastconv((const float2 *)x, ldx, spec, 0, (float2 *)y, ldy, rows, p->scale, s);
 }
 
 // ---------------------------------------------------------------- plans

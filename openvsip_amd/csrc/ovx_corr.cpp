@@ -15,6 +15,7 @@
 #include <new>
 #include <vector>
 
+#include "ovx_hier.hpp"
 #include "ovx_ols.hpp"
 #include "ovx_plan.hpp"
 
@@ -43,8 +44,12 @@ static int ols_init(CorrState &c)
 {
   c.L = choose_L(c.M);
   if (c.M - 1 >= c.L) return fail(OVX_ERR_UNSUPPORTED, "kernel length %ld too long for overlap-save", c.M);
-  int st = c.fftL.init(c.L, 0);
+  // the overlap-save kernel is built on the register-Stockham engine: take
+  // that path's configuration (variant 0 of it) and its twiddle table
+  int st = c.fftL.init(c.L, hier_supported(c.L) ? 1 : 0);
   if (st) return st;
+  if (c.fftL.kind != LineFFT::LDS || !c.fftL.tw.p)
+    return fail(OVX_ERR_UNSUPPORTED, "no register-Stockham configuration for segment length %d", c.L);
   if ((st = c.hpad.alloc((size_t)c.L * 8))) return st;
   if ((st = c.H.alloc((size_t)c.L * 8))) return st;
   return OVX_OK;
@@ -60,6 +65,8 @@ static int ols_spectrum(CorrState &c, const float2 *h, int reverse, hipStream_t 
 
 static int ols_run(CorrState &c, const float2 *in, float2 *out, int conj_input, int unbiased, hipStream_t s)
 {
+  if (c.fftL.kind != LineFFT::LDS || !c.fftL.tw.p || !c.H.p)
+    return fail(OVX_ERR_STATE, "overlap-save plan has no twiddle table");
   OlsArgs a;
   a.in = in;
   a.N = c.N;

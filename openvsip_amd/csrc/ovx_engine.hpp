@@ -66,10 +66,41 @@ constexpr RadixPlan make_plan(int N, int P)
 }
 
 constexpr int lds_pad(int pos) { return pos + (pos >> 4); }
+
+// Pointer into LDS with the address space spelled out (for code that is not
+// inlined into the kernel owning the __shared__ array), and element access
+// helpers that work for it and for a plain float2 pointer alike.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) f2v lds_f2v;
+typedef lds_f2v *lds_ptr;
+__device__ __forceinline__ lds_ptr as_lds(float2 *p) { return (lds_ptr)p; }
+__device__ __forceinline__ void lds_st(float2 *p, int i, float2 v) { p[i] = v; }
+__device__ __forceinline__ float2 lds_ld(const float2 *p, int i) { return p[i]; }
+__device__ __forceinline__ void lds_st(lds_ptr p, int i, float2 v) { p[i] = f2v{ v.x, v.y }; }
+__device__ __forceinline__ float2 lds_ld(lds_ptr p, int i)
+{
+  f2v t = p[i];
+  return make_float2(t.x, t.y);
+}
 constexpr int lds_row_elems(int N) { return N >= 16 ? N + N / 16 : N; }
 
+// Synchronisation between the LDS write and read halves of an exchange.
+// Block scope: every wave of the workgroup takes part in the line.
+// Wave scope: the line lives in one wavefront, whose LDS operations complete
+// in order; waiting for them (lgkmcnt 0) is enough, no workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void exchange_sync()
+{
+  if constexpr (WAVE) {
+    __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0), vmcnt/expcnt untouched
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- engine
-template <int N, int P>
+template <int N, int P, bool WAVE = false>
 struct Engine {
   static constexpr int T = N / P;
   static constexpr RadixPlan plan = make_plan(N, P);
@@ -80,8 +111,8 @@ struct Engine {
 
   // One stage.  tw: concatenated forward twiddle table (FP32 values of FP64
   // W_{Ns R}^{r k}, layout tw[twoff_s + (r-1)*Ns + k]).
-  template <int S, int DIR>
-  __device__ __forceinline__ static void stage(float2 (&v)[P], float2 *lds, int t, rsrc_t tw)
+  template <int S, int DIR, typename LP>
+  __device__ __forceinline__ static void stage(float2 (&v)[P], LP lds, int t, rsrc_t tw)
   {
     constexpr int R = plan.radix[S];
     constexpr int Ns = plan.ns[S];
@@ -108,19 +139,19 @@ struct Engine {
       } else {
         const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[lds_pad(base + r * Ns)] = a[r];
+        for (int r = 0; r < R; ++r) lds_st(lds, lds_pad(base + r * Ns), a[r]);
       }
     }
     if constexpr (!last) {
-      __syncthreads();
+      exchange_sync<WAVE>();
 #pragma unroll
-      for (int q = 0; q < P; ++q) v[q] = lds[lds_pad(t + T * q)];
-      __syncthreads();
+      for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, lds_pad(t + T * q));
+      exchange_sync<WAVE>();
     }
   }
 
-  template <int S, int DIR>
-  __device__ __forceinline__ static void stages_from(float2 (&v)[P], float2 *lds, int t, rsrc_t tw)
+  template <int S, int DIR, typename LP>
+  __device__ __forceinline__ static void stages_from(float2 (&v)[P], LP lds, int t, rsrc_t tw)
   {
     if constexpr (S < NST) {
       stage<S, DIR>(v, lds, t, tw);
@@ -129,8 +160,10 @@ struct Engine {
   }
 
   // Full transform of the line held in slot form; result in slot form.
-  template <int DIR>
-  __device__ __forceinline__ static void fft(float2 (&v)[P], float2 *lds, int t, rsrc_t tw)
+  // LP: pointer to the line's LDS region (float2*, or an LDS address-space
+  // pointer when the caller is not inlined into the kernel that owns it).
+  template <int DIR, typename LP>
+  __device__ __forceinline__ static void fft(float2 (&v)[P], LP lds, int t, rsrc_t tw)
   {
     stages_from<0, DIR>(v, lds, t, tw);
   }

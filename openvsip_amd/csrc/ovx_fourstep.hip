@@ -18,127 +18,22 @@
 // fastest across lanes, so every global access is a C*8-byte contiguous run.
 // W_N^m is rebuilt from two small FP64-generated tables:
 // W_N^m = Thi[m >> S] * Tlo[m & (2^S - 1)].
-#include "ovx_fourstep.hpp"
-#include "ovx_internal.hpp"
+#include "ovx_fs_dev.hpp"
 
 namespace ovx {
 
-// LDS stride per line: odd number of float2 so that lanes that differ in the
-// line index land on different banks.
-template <int L>
-struct FsLds {
-  static constexpr int LROW = lds_row_elems(L) | 1;
-};
-
-__device__ __forceinline__ float2 big_twiddle(const float2 *__restrict__ tlo, const float2 *__restrict__ thi,
-                                              int sbits, uint32_t m)
-{
-  float2 lo = tlo[m & ((1u << sbits) - 1u)];
-  float2 hi = thi[m >> sbits];
-  return cmul(hi, lo);
-}
-
-// ---------------------------------------------------------------- column pass
-// MODE 0: forward column FFT, then * W_N^{n2 k1}      (fastconv / Fftm pass 1)
-// MODE 1: inverse column FFT, then * scale             (fastconv pass 3)
-// MODE 2: inverse column FFT, then * W_N^{-n2 k1}      (inverse Fftm pass 1)
 template <int L, int P, int C, int MODE>
 __global__ __launch_bounds__(C *(L / P)) void k_fs_col(FsArgs a)
 {
-  using E = Engine<L, P>;
-  constexpr int T = E::T;
-  constexpr int LROW = FsLds<L>::LROW;
-  __shared__ float2 lds_all[C * LROW];
-  const int tid = threadIdx.x;
-  const int c = tid % C;
-  const int t = tid / C;
-  const long row = blockIdx.y;
-  const long n2 = (long)blockIdx.x * C + c;
-  const long N2 = a.n2;
-  const float2 *src = a.in + row * a.ld_in + n2;
-  float2 *dst = a.out + row * a.ld_out + n2;
-  const rsrc_t rtw = make_rsrc(a.tw1, (uint32_t)(E::plan.twtotal + 1) * 8u);
-  float2 *lds = lds_all + c * LROW;
-
-  float2 v[P];
-#pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = src[(long)(t + T * q) * N2];
-  if constexpr (MODE == 0) E::template fft<-1>(v, lds, t, rtw);
-  else E::template fft<+1>(v, lds, t, rtw);
-  if constexpr (MODE == 0 || MODE == 2) {
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-      const uint32_t k1 = (uint32_t)(t + T * q);
-      const uint32_t m = (uint32_t)n2 * k1; // < N1*N2: no reduction needed
-      float2 w = big_twiddle(a.tlo, a.thi, a.sbits, m);
-      v[q] = (MODE == 0) ? cmul(v[q], w) : cmulc(v[q], w);
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < P; ++q) v[q] = cscale(v[q], a.scale);
-  }
-#pragma unroll
-  for (int q = 0; q < P; ++q) dst[(long)(t + T * q) * N2] = v[q];
+  __shared__ float2 lds_all[C * FsLds<L>::LROW];
+  fs_col_body<L, P, C, MODE>(a, blockIdx.y, blockIdx.x, threadIdx.x, lds_all);
 }
 
-// ---------------------------------------------------------------- row pass
-// MODE 0: fastconv middle pass (in place allowed):
-//         forward row FFT, * Wp[k1][k2], inverse row FFT, * W_N^{-n2 k1}
-// MODE 1: forward row FFT, transposed store X[k1 + N1 k2], * scale
-// MODE 2: inverse row FFT, transposed store,               * scale
 template <int L, int P, int C, int MODE>
 __global__ __launch_bounds__(C *(L / P)) void k_fs_row(FsArgs a)
 {
-  using E = Engine<L, P>;
-  constexpr int T = E::T;
-  constexpr int LROW = FsLds<L>::LROW;
-  __shared__ float2 lds_all[C * LROW];
-  const int tid = threadIdx.x;
-  const int c = tid / T; // line fastest-varying within a line: t
-  const int t = tid % T;
-  const long row = blockIdx.y;
-  const long k1 = (long)blockIdx.x * C + c;
-  const long N1 = a.n1;
-  const rsrc_t rtw = make_rsrc(a.tw2, (uint32_t)(E::plan.twtotal + 1) * 8u);
-  float2 *lds = lds_all + c * LROW;
-  const float2 *src = a.in + row * a.ld_in + k1 * L;
-
-  float2 v[P];
-#pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = src[t + T * q];
-  if constexpr (MODE == 0) {
-    E::template fft<-1>(v, lds, t, rtw);
-    const float2 *wp = a.wp + row * a.ldwp + k1 * L;
-#pragma unroll
-    for (int q = 0; q < P; ++q) v[q] = cmul(v[q], wp[t + T * q]);
-    E::template fft<+1>(v, lds, t, rtw);
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-      const uint32_t n2 = (uint32_t)(t + T * q);
-      const uint32_t m = n2 * (uint32_t)k1;
-      v[q] = cmulc(v[q], big_twiddle(a.tlo, a.thi, a.sbits, m));
-    }
-    float2 *dst = a.out + row * a.ld_out + k1 * L;
-#pragma unroll
-    for (int q = 0; q < P; ++q) dst[t + T * q] = v[q];
-  } else {
-    if constexpr (MODE == 1) E::template fft<-1>(v, lds, t, rtw);
-    else E::template fft<+1>(v, lds, t, rtw);
-    if (a.scale != 1.0f) {
-#pragma unroll
-      for (int q = 0; q < P; ++q) v[q] = cscale(v[q], a.scale);
-    }
-    // transpose through LDS: element k2 of line c -> out[k1_0 + c + N1 k2]
-#pragma unroll
-    for (int q = 0; q < P; ++q) lds[t + T * q] = v[q];
-    __syncthreads();
-    const int c2 = tid % C;
-    const int t2 = tid / C;
-    const float2 *lsrc = lds_all + c2 * LROW;
-    float2 *dst = a.out + row * a.ld_out + (long)blockIdx.x * C + c2;
-#pragma unroll
-    for (int q = 0; q < P; ++q) dst[(long)(t2 + T * q) * N1] = lsrc[t2 + T * q];
-  }
+  __shared__ float2 lds_all[C * FsLds<L>::LROW];
+  fs_row_body<L, P, C, MODE>(a, blockIdx.y, blockIdx.x, threadIdx.x, lds_all);
 }
 
 // Wp[k1][k2] = scale * W[k1 + N1 k2] for `rows` coefficient rows (ldw = 0:
@@ -156,12 +51,6 @@ __global__ void k_fs_permute(const float2 *__restrict__ w, long ldw, float2 *__r
 }
 
 // ---------------------------------------------------------------- dispatch
-template <int L>
-struct FsCfg {
-  static constexpr int P = 16;
-  static constexpr int C = (L / P) * 16 <= 1024 ? 16 : 1024 / (L / P);
-  static constexpr int NT = C * (L / P);
-};
 
 template <int L, int MODE>
 static hipError_t col_launch(const FsArgs &a, long rows, hipStream_t s)

@@ -1,0 +1,256 @@
+// ovx_hier_dev.hpp -- device bodies of the hierarchical transforms for
+// N = 1024 * {1, 2, 4, 8, 16}; see ovx_hier.hip for the algorithm.
+#pragma once
+#include "ovx_hier.hpp"
+#include "ovx_internal.hpp"
+
+namespace ovx {
+
+template <int N>
+struct Hier {
+  static constexpr int NW = 1024;        // points per wave transform
+  static constexpr int R0 = N / NW;      // outer radix = waves per row
+  static constexpr int TR = 64 * R0;     // threads per row
+  static constexpr int M0 = 16 / R0;     // outer butterflies per thread
+  static constexpr int LSL = NW + NW / 16; // LDS slice stride (padded)
+  static constexpr int LROW = R0 * LSL;  // = N + N/16
+  static constexpr int B = TR >= 256 ? 1 : 256 / TR; // rows per block
+  static constexpr int NT = TR * B;
+  using WE = Engine<NW, 16, true>;
+  static_assert(R0 * NW == N && R0 >= 1 && R0 <= 16, "N must be 1024 * {1,2,4,8,16}");
+};
+
+// Outer forward stage: load, DFT_R0, twiddle, write slices.
+template <int N, typename LP>
+__device__ __forceinline__ void outer_forward(const rsrc_t &rx, uint32_t vx, bool valid, LP lds,
+                                              const float2 *__restrict__ two, int t)
+{
+  using H = Hier<N>;
+  constexpr int R0 = H::R0, M0 = H::M0, NW = H::NW;
+#pragma unroll
+  for (int m = 0; m < M0; ++m) {
+    const int j = t + H::TR * m;
+    float2 a[R0];
+#pragma unroll
+    for (int r = 0; r < R0; ++r)
+      a[r] = valid ? bload_nt(rx, vx, (uint32_t)(H::TR * m + NW * r) * 8u) : make_float2(0.f, 0.f);
+    if constexpr (R0 > 1) {
+      Dft<R0, -1>::run(a);
+#pragma unroll
+      for (int k1 = 1; k1 < R0; ++k1) a[k1] = cmul(a[k1], two[(k1 - 1) * NW + j]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R0; ++k1) lds_st(lds, k1 * H::LSL + lds_pad(j), a[k1]);
+  }
+}
+
+// Outer inverse stage: read slices, DFT_R0 (inverse), scale, store.
+template <int N, typename LP>
+__device__ __forceinline__ void outer_inverse_store(const rsrc_t &ry, uint32_t vy, bool valid, LP lds,
+                                                    int t, float scale)
+{
+  using H = Hier<N>;
+  constexpr int R0 = H::R0, M0 = H::M0, NW = H::NW;
+#pragma unroll
+  for (int m = 0; m < M0; ++m) {
+    const int j = t + H::TR * m;
+    float2 a[R0];
+#pragma unroll
+    for (int k1 = 0; k1 < R0; ++k1) a[k1] = lds_ld(lds, k1 * H::LSL + lds_pad(j));
+    if constexpr (R0 > 1) Dft<R0, +1>::run(a);
+    if (scale != 1.0f) {
+#pragma unroll
+      for (int r = 0; r < R0; ++r) a[r] = cscale(a[r], scale);
+    }
+    if (valid) {
+#pragma unroll
+      for (int r = 0; r < R0; ++r) bstore_nt(ry, a[r], vy, (uint32_t)(H::TR * m + NW * r) * 8u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- fastconv
+// w_perm: spectrum in [k1][k2] order (w_perm[k1*1024 + k2] = W[k1 + R0 k2]),
+// one vector (ldw = 0) or one per row; when natural != 0 the spectrum is in
+// natural order and gathered with stride R0 instead.
+// Body shared by the standalone kernel and the persistent batch kernel:
+// rows row0 + lrow of the job, thread t of the row, LDS row region `lds`.
+// Every thread of the workgroup must call it (it contains workgroup
+// barriers); rows at or beyond a.rows compute on zeros and store nothing.
+template <int N, typename LP>
+__device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0, int lrow, int t, LP lds)
+{
+  using H = Hier<N>;
+  constexpr int R0 = H::R0, NW = H::NW;
+  const long row = row0 + lrow;
+  const bool valid = row < a.rows;
+  const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
+  const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
+  const rsrc_t rw = make_rsrc(a.w + row0 * a.ldw, 0xffffffffu);
+  const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
+  const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
+  const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
+
+  if constexpr (R0 > 1) {
+    outer_forward<N, LP>(rx, vx, valid, lds, a.tw_outer, t);
+    __syncthreads();
+  }
+  // ---- wave stage: wave k1 owns slice k1
+  const int k1 = t >> 6;
+  const int l = t & 63;
+  LP sl = lds + k1 * H::LSL;
+  float2 u[16];
+  if constexpr (R0 > 1) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, lds_pad(l + 64 * s));
+  } else {
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      u[s] = valid ? bload_nt(rx, vx, (uint32_t)(64 * s) * 8u) : make_float2(0.f, 0.f);
+  }
+  H::WE::template fft<-1>(u, sl, l, rtw);
+  if (a.natural) {
+    const uint32_t vw = (uint32_t)(lrow * a.ldw + k1 + R0 * l) * 8u;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      float2 ww = bload(rw, vw, (uint32_t)(R0 * 64 * s) * 8u);
+      u[s] = cmul(u[s], ww);
+    }
+  } else {
+    const uint32_t vw = (uint32_t)(lrow * a.ldw + k1 * NW + l) * 8u;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      float2 ww = bload(rw, vw, (uint32_t)(64 * s) * 8u);
+      u[s] = cmul(u[s], ww);
+    }
+  }
+  if (a.scale != 1.0f) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) u[s] = cscale(u[s], a.scale);
+  }
+  H::WE::template fft<+1>(u, sl, l, rtw);
+  if constexpr (R0 > 1) {
+    if (k1 > 0) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) u[s] = cmulc(u[s], a.tw_outer[(k1 - 1) * NW + l + 64 * s]);
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s) lds_st(sl, lds_pad(l + 64 * s), u[s]);
+    __syncthreads();
+    outer_inverse_store<N, LP>(ry, vy, valid, lds, t, 1.0f);
+  } else {
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) bstore_nt(ry, u[s], vy, (uint32_t)(64 * s) * 8u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- transforms
+// Forward: outer stage, wave FFT, then the natural-order result
+// X[k1 + R0 k2] is staged through LDS so that the store is coalesced.
+template <int N, typename LP>
+__device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int lrow, int t, LP lds)
+{
+  using H = Hier<N>;
+  constexpr int R0 = H::R0;
+  const long row = row0 + lrow;
+  const bool valid = row < a.rows;
+  const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
+  const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
+  const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
+  const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
+  const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
+
+  if constexpr (R0 > 1) {
+    outer_forward<N, LP>(rx, vx, valid, lds, a.tw_outer, t);
+    __syncthreads();
+  }
+  const int k1 = t >> 6;
+  const int l = t & 63;
+  LP sl = lds + k1 * H::LSL;
+  float2 u[16];
+  if constexpr (R0 > 1) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, lds_pad(l + 64 * s));
+  } else {
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      u[s] = valid ? bload_nt(rx, vx, (uint32_t)(64 * s) * 8u) : make_float2(0.f, 0.f);
+  }
+  H::WE::template fft<-1>(u, sl, l, rtw);
+  if (a.scale != 1.0f) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) u[s] = cscale(u[s], a.scale);
+  }
+  if constexpr (R0 > 1) {
+    __syncthreads(); // every wave is done with its slice
+#pragma unroll
+    for (int s = 0; s < 16; ++s) lds_st(lds, lds_pad(k1 + R0 * (l + 64 * s)), u[s]);
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) bstore_nt(ry, lds_ld(lds, lds_pad(t + H::TR * q)), vy, (uint32_t)(H::TR * q) * 8u);
+    }
+  } else {
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) bstore_nt(ry, u[s], vy, (uint32_t)(64 * s) * 8u);
+    }
+  }
+}
+
+// Inverse: natural-order input staged through LDS into [k1][k2] slices, wave
+// inverse FFT, inverse twiddle, outer inverse stage.
+template <int N, typename LP>
+__device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int lrow, int t, LP lds)
+{
+  using H = Hier<N>;
+  constexpr int R0 = H::R0, NW = H::NW;
+  const long row = row0 + lrow;
+  const bool valid = row < a.rows;
+  const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
+  const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
+  const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
+  const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
+  const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
+  const int k1 = t >> 6;
+  const int l = t & 63;
+  LP sl = lds + k1 * H::LSL;
+  float2 u[16];
+  if constexpr (R0 > 1) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      lds_st(lds, lds_pad(t + H::TR * q), valid ? bload_nt(rx, vx, (uint32_t)(H::TR * q) * 8u) : make_float2(0.f, 0.f));
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 16; ++s) u[s] = lds_ld(lds, lds_pad(k1 + R0 * (l + 64 * s)));
+    __syncthreads(); // all natural-order reads done before slices are reused
+  } else {
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      u[s] = valid ? bload_nt(rx, vx, (uint32_t)(64 * s) * 8u) : make_float2(0.f, 0.f);
+  }
+  H::WE::template fft<+1>(u, sl, l, rtw);
+  if constexpr (R0 > 1) {
+    if (k1 > 0) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) u[s] = cmulc(u[s], a.tw_outer[(k1 - 1) * NW + l + 64 * s]);
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s) lds_st(sl, lds_pad(l + 64 * s), u[s]);
+    __syncthreads();
+    outer_inverse_store<N, LP>(ry, vy, valid, lds, t, a.scale);
+  } else {
+    if (a.scale != 1.0f) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) u[s] = cscale(u[s], a.scale);
+    }
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) bstore_nt(ry, u[s], vy, (uint32_t)(64 * s) * 8u);
+    }
+  }
+}
+
+} // namespace ovx

@@ -44,6 +44,9 @@ hipError_t launch_row_fastconv(const RowConfig &cfg, const float2 *x, long ldx,
 // exp(-2 pi i r k / (Ns R_s)) evaluated in FP64 on the device, rounded to FP32.
 hipError_t launch_twiddles(const RadixPlan &plan, float2 *tw, hipStream_t s);
 
+// out[i] = scale * in[i], i < n.
+hipError_t launch_scale_copy(const float2 *in, float2 *out, long n, float scale, hipStream_t s);
+
 // Table of exp(-2 pi i j / n), j in [0, count), FP64-evaluated, FP32-rounded.
 hipError_t launch_unit_roots(long n, long count, float2 *out, hipStream_t s);
 

@@ -28,10 +28,13 @@ struct DevBuf {
 // A line transform of length n: picks the kernel path (LDS-resident rows,
 // or the two-pass four-step for longer lines) and owns its twiddles.
 struct LineFFT {
-  enum Kind { NONE, LDS, FOURSTEP } kind = NONE;
+  enum Kind { NONE, LDS, FOURSTEP, HIER } kind = NONE;
   long n = 0;
+  int variant = 0;
   RowConfig rc;
   DevBuf tw;
+  // hierarchical (outer stage + per-wave 1024-point transforms)
+  DevBuf tw_outer, tw_wave;
   // two-pass four-step (n = n1 * n2)
   long n1 = 0, n2 = 0;
   int sbits = 0;
@@ -47,6 +50,15 @@ struct LineFFT {
                 hipStream_t s) const;
   int fastconv(const float2 *x, long ldx, const float2 *w, long ldw, float2 *y, long ldy, long lines, float scale,
                hipStream_t s) const;
+  // Spectrum layout the fastconv kernels of this path read: wp = scale * w
+  // permuted for the path (n values).  fastconv_prepared() takes such a
+  // spectrum (one vector for all lines) and applies no further scale.
+  int prepare_spectrum(const float2 *w, float2 *wp, float scale, hipStream_t s) const;
+  int fastconv_prepared(const float2 *x, long ldx, const float2 *wp, float2 *y, long ldy, long lines,
+                        hipStream_t s) const;
+  static int variants(long n);
+  // every table the kernels of this path read has been allocated
+  bool tables_ready() const;
 };
 
 enum PlanKind { PLAN_FFTM = 1, PLAN_FFT, PLAN_FASTCONV, PLAN_CORR, PLAN_CONV };

@@ -8,6 +8,7 @@
 //   scale pass           src/ovxx/signal/fft/workspace.hpp:474-478
 //   vmmul<0> row loop    src/vsip/impl/vmmul.hpp:79-85
 #include "ovx_internal.hpp"
+#include "ovx_row_dev.hpp"
 
 namespace ovx {
 
@@ -25,39 +26,12 @@ __global__ __launch_bounds__(N / P * B) void k_row_fft(const float2 *__restrict_
                                                       float2 *__restrict__ out, long ld_out, long rows,
                                                       const float2 *__restrict__ tw, float scale)
 {
-  using E = Engine<N, P>;
-  constexpr int T = E::T;
+  using E = Engine<N, P, (N / P <= 64)>;
   __shared__ float2 lds_all[B * E::LROW];
-  const int tid = threadIdx.x;
-  const int lrow = (B == 1) ? 0 : tid / T;
-  const int t = (B == 1) ? tid : tid % T;
-  const long row = (long)blockIdx.x * B + lrow;
-  const bool valid = row < rows;
-  float2 *lds = lds_all + lrow * E::LROW;
-
-  const rsrc_t rin = make_rsrc(in + (long)blockIdx.x * B * ld_in, 0xffffffffu);
-  const rsrc_t rout = make_rsrc(out + (long)blockIdx.x * B * ld_out, 0xffffffffu);
-  const rsrc_t rtw = make_rsrc(tw, (uint32_t)(E::plan.twtotal + 1) * 8u);
-  const uint32_t vin = (uint32_t)(lrow * ld_in + t) * 8u;
-  const uint32_t vout = (uint32_t)(lrow * ld_out + t) * 8u;
-
-  float2 v[P];
-  if (valid) {
-#pragma unroll
-    for (int q = 0; q < P; ++q) v[q] = bload_nt(rin, vin, (uint32_t)(T * q) * 8u);
-  } else {
-#pragma unroll
-    for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
-  }
-  E::template fft<DIR>(v, lds, t, rtw);
-  if (scale != 1.0f) {
-#pragma unroll
-    for (int q = 0; q < P; ++q) v[q] = cscale(v[q], scale);
-  }
-  if (valid) {
-#pragma unroll
-    for (int q = 0; q < P; ++q) bstore_nt(rout, v[q], vout, (uint32_t)(T * q) * 8u);
-  }
+  const int lrow = (B == 1) ? 0 : (int)threadIdx.x / E::T;
+  const int t = (B == 1) ? (int)threadIdx.x : (int)threadIdx.x % E::T;
+  row_fft_body<N, P, DIR>(in, ld_in, out, ld_out, rows, tw, scale, (long)blockIdx.x * B, lrow, t,
+                          lds_all + lrow * E::LROW);
 }
 
 template <int N, int P, int B>
@@ -66,44 +40,13 @@ __global__ __launch_bounds__(N / P * B) void k_row_fastconv(const float2 *__rest
                                                            float2 *__restrict__ y, long ldy, long rows,
                                                            const float2 *__restrict__ tw, float scale)
 {
-  using E = Engine<N, P>;
+  using E = Engine<N, P, (N / P <= 64)>;
   constexpr int T = E::T;
   __shared__ float2 lds_all[B * E::LROW];
-  const int tid = threadIdx.x;
-  const int lrow = (B == 1) ? 0 : tid / T;
-  const int t = (B == 1) ? tid : tid % T;
-  const long row = (long)blockIdx.x * B + lrow;
-  const bool valid = row < rows;
-  float2 *lds = lds_all + lrow * E::LROW;
-
-  const rsrc_t rx = make_rsrc(x + (long)blockIdx.x * B * ldx, 0xffffffffu);
-  const rsrc_t ry = make_rsrc(y + (long)blockIdx.x * B * ldy, 0xffffffffu);
-  const rsrc_t rw = make_rsrc(w + (long)blockIdx.x * B * ldw, 0xffffffffu);
-  const rsrc_t rtw = make_rsrc(tw, (uint32_t)(E::plan.twtotal + 1) * 8u);
-  const uint32_t vx = (uint32_t)(lrow * ldx + t) * 8u;
-  const uint32_t vy = (uint32_t)(lrow * ldy + t) * 8u;
-  const uint32_t vw = (uint32_t)(lrow * ldw + t) * 8u;
-
-  float2 v[P];
-  if (valid) {
-#pragma unroll
-    for (int q = 0; q < P; ++q) v[q] = bload_nt(rx, vx, (uint32_t)(T * q) * 8u);
-  } else {
-#pragma unroll
-    for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
-  }
-  E::template fft<-1>(v, lds, t, rtw);
-  // multiply by the (scaled) replica spectrum; slot q holds frequency t + T q
-#pragma unroll
-  for (int q = 0; q < P; ++q) {
-    float2 ww = bload(rw, vw, (uint32_t)(T * q) * 8u);
-    v[q] = cmul(v[q], cscale(ww, scale));
-  }
-  E::template fft<+1>(v, lds, t, rtw);
-  if (valid) {
-#pragma unroll
-    for (int q = 0; q < P; ++q) bstore_nt(ry, v[q], vy, (uint32_t)(T * q) * 8u);
-  }
+  const int lrow = (B == 1) ? 0 : (int)threadIdx.x / T;
+  const int t = (B == 1) ? (int)threadIdx.x : (int)threadIdx.x % T;
+  row_fastconv_body<N, P>(x, ldx, w, ldw, y, ldy, rows, tw, scale, (long)blockIdx.x * B, lrow, t,
+                          lds_all + lrow * E::LROW);
 }
 
 // ---------------------------------------------------------------- twiddles
@@ -151,6 +94,19 @@ hipError_t launch_twiddles(const RadixPlan &plan, float2 *tw, hipStream_t s)
   for (int k = 0; k < plan.nst; ++k) { d.radix[k] = plan.radix[k]; d.ns[k] = plan.ns[k]; d.off[k] = plan.twoff[k]; }
   int blocks = (plan.twtotal + 255) / 256;
   hipLaunchKernelGGL(k_twiddles, dim3(blocks), dim3(256), 0, s, d, tw, plan.twtotal);
+  return hipGetLastError();
+}
+
+__global__ void k_scale_copy(const float2 *__restrict__ in, float2 *__restrict__ out, long n, float scale)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = cscale(in[i], scale);
+}
+
+hipError_t launch_scale_copy(const float2 *in, float2 *out, long n, float scale, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n, scale);
   return hipGetLastError();
 }
 
@@ -215,6 +171,7 @@ static hipError_t do_fastconv(const float2 *x, long ldx, const float2 *w, long l
   X(512, 16) X(1024, 16) X(2048, 16) X(4096, 16) X(8192, 16) X(16384, 16)       \
   X(3, 3) X(6, 6) X(12, 12) X(24, 24) X(48, 24) X(96, 24) X(192, 24)            \
   X(384, 24) X(768, 24) X(1536, 24) X(3072, 24) X(6144, 24) X(12288, 24)       \
+  X(3072, 12) X(6144, 12) X(12288, 12)                                          \
   X(5, 5) X(10, 10) X(20, 20) X(40, 40) X(80, 40) X(160, 40) X(320, 40)         \
   X(640, 40) X(1280, 40) X(2560, 40) X(5120, 40) X(10240, 40)
 

@@ -293,3 +293,42 @@ def test_fastconv_fourstep_coefficient_matrix(ovx):
     fc = ovx.Fastconv(rows, n)
     y = host(fc(dev(x), dev(c)))
     assert o.rel_l2(y, o.fastconv64(x, c)) < REL_L2
+
+
+# ---- kernel variants and the resident-replica path ----------------------------
+
+@pytest.mark.parametrize("n", [1024, 2048, 4096, 8192, 16384])
+def test_every_variant_matches_fp64(ovx, n):
+    """Each compiled kernel variant of a length (the autotuner's choices) gives
+    the same result to the stated tolerance: fastconv, forward and inverse."""
+    rows = 21
+    rng = np.random.default_rng(n + 5)
+    x = cg(rng, (rows, n))
+    w = cg(rng, n)
+    c = cg(rng, (rows, n))
+    ref = o.fastconv64(x, w)
+    refc = o.fastconv64(x, c)
+    reff = o.fftm64(x.astype(np.complex128), o.FWD)
+    refi = o.fftm64(x.astype(np.complex128), o.INV, 1.0 / n)
+    for v in range(ovx.variants(n)):
+        say("variant", n, v)
+        fc = ovx.Fastconv(rows, n, variant=v)
+        assert o.rel_l2(host(fc(dev(x), dev(w))), ref) < REL_L2, (n, v, fc.name)
+        assert o.rel_l2(host(fc(dev(x), dev(c))), refc) < REL_L2, (n, v, fc.name)
+        ff = ovx.Fftm(rows, n, ovx.ROW, ovx.FFT_FWD, 1.0, variant=v)
+        assert o.rel_l2(host(ff(dev(x))), reff) < REL_L2, (n, v, ff.name)
+        fi = ovx.Fftm(rows, n, ovx.ROW, ovx.FFT_INV, 1.0 / n, variant=v)
+        assert o.rel_l2(host(fi(dev(x))), refi) < REL_L2, (n, v, fi.name)
+
+
+@pytest.mark.parametrize("n", [512, 2048, 3072, 16384, 65536])
+def test_fastconv_resident_replica(ovx, n):
+    rows = 9
+    rng = np.random.default_rng(n + 7)
+    x = cg(rng, (rows, n))
+    chirp = wl.lfm_chirp(n // 4, n // 32).astype(np.complex64)
+    rep = ovx.Replica(n, chirp)
+    fc = ovx.Fastconv(rows, n)
+    y = host(fc.with_replica(dev(x), rep))
+    w = wl.replica_spectrum(chirp.astype(np.complex128), n)
+    assert o.rel_l2(y, o.fastconv64(x, w)) < REL_L2

@@ -1,0 +1,101 @@
+"""Continuous-batching server: a mixed stream of fastconv, forward Fftm and
+correlation requests.  Every result is bitwise the same through the batched
+server and through the plain one-launch-per-request server (the same device
+bodies run in both), and matches the FP64 oracle to rel-L2 <= 1e-5."""
+import numpy as np
+import pytest
+
+import oracle as o
+from openvsip_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def build_mix(ovx):
+    import torch
+    rng = np.random.default_rng(11)
+    specs = [("fc", 512, 40), ("fc", 2048, 17), ("fc", 3072, 9), ("fc", 16384, 5), ("fc", 65536, 3),
+             ("fc", 12288, 6), ("fft", 1024, 12), ("fft", 6144, 7), ("fft", 65536, 2), ("fft", 512, 33),
+             ("corr", 65536, 1), ("fc", 8192, 30), ("fft", 16384, 4), ("corr", 65536, 1), ("fc", 4096, 64)]
+    reps, items = {}, []
+    ref = wl.cgauss(rng, 1024)
+    cref = ovx.CorrReference("full", ref, 65536)
+    for i, (kind, n, rows) in enumerate(specs):
+        if kind == "corr":
+            x = wl.cgauss(rng, n)
+            items.append(dict(kind=kind, n=n, x=x, xd=dev(x), bias="unbiased" if i % 2 else "biased",
+                              ref=ref, cref=cref))
+            continue
+        x = wl.cgauss(rng, (rows, n))
+        it = dict(kind=kind, n=n, rows=rows, x=x, xd=dev(x))
+        if kind == "fc":
+            if n not in reps:
+                w = wl.replica_spectrum(wl.lfm_chirp(n, n // 8), n)
+                reps[n] = (ovx.Replica(n, w, is_spectrum=True), w)
+            it["rep"] = reps[n]
+        items.append(it)
+    return items, reps, cref
+
+
+def run(ovx, items, plain):
+    import torch
+    srv = ovx.Server(slots=3, max_jobs=8, hbm_budget_mb=4096, workspace_mb=256, plain=plain)
+    outs, tickets = [], []
+    for it in items:
+        if it["kind"] == "corr":
+            y = torch.full((it["cref"].output_size,), float("nan"), dtype=torch.complex64, device="cuda")
+            req = ovx.corr_request(it["xd"], it["cref"], y, it["bias"])
+        else:
+            y = torch.full_like(it["xd"], float("nan"))
+            req = ovx.fastconv_request(it["xd"], it["rep"][0], y) if it["kind"] == "fc" else \
+                ovx.fftm_request(it["xd"], y)
+        tickets.append(srv.submit(req))
+        outs.append(y)
+    lats = [srv.wait(t) for t in tickets]
+    res = [host(y) for y in outs]
+    srv.close()
+    return res, lats
+
+
+def test_server_mix_same_alone_and_in_mix_and_vs_oracle(ovx):
+    items, reps, cref = build_mix(ovx)
+    batched, lat_b = run(ovx, items, plain=False)
+    plain, lat_p = run(ovx, items, plain=True)
+    for it, yb, yp in zip(items, batched, plain):
+        print(it["kind"], it["n"], it.get("rows"), flush=True)
+        assert np.isfinite(yb).all()
+        assert np.array_equal(yb, yp), (it["kind"], it["n"])
+        if it["kind"] == "fc":
+            ref = o.fastconv64(it["x"], it["rep"][1])
+        elif it["kind"] == "fft":
+            ref = o.fftm64(it["x"].astype(np.complex128), o.FWD)
+        else:
+            ref = o.corr64(it["bias"], "full", it["ref"], it["x"])
+        assert o.rel_l2(yb, ref) < 1e-5, (it["kind"], it["n"])
+    assert all(l >= 0 for l in lat_b + lat_p)
+
+
+def test_server_refuses_over_budget(ovx):
+    import torch
+    from openvsip_amd import OvxError
+    srv = ovx.Server(slots=1, max_jobs=4, hbm_budget_mb=1, workspace_mb=16)
+    x = torch.zeros((64, 4096), dtype=torch.complex64, device="cuda")  # 2 MiB + 2 MiB > 1 MiB
+    y = torch.empty_like(x)
+    w = wl.replica_spectrum(wl.lfm_chirp(4096, 512), 4096)
+    rep = ovx.Replica(4096, w, is_spectrum=True)
+    t = srv.submit(ovx.fastconv_request(x, rep, y))
+    with pytest.raises(OvxError) as ei:
+        srv.wait(t)
+    assert ei.value.code == 5  # OVX_ERR_BUDGET
+    srv.close()
