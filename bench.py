@@ -38,6 +38,13 @@ def parse():
     ap.add_argument("--variant", type=int, default=None, help="force a compiled kernel variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-jobs measurement")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--no-precision", action="store_true", help="skip the BF16x3 MFMA variant measurement")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="time a plain device copy of the workload's bytes instead (reference for HBM counters)")
+    ap.add_argument("--serve-rates", default="2000,5000,10000,20000", help="offered rates, jobs/s")
+    ap.add_argument("--serve-seconds", type=float, default=0.5, help="arrival window per rate")
+    ap.add_argument("--serve-saturation-jobs", type=int, default=3000)
     return ap.parse_args()
 
 
@@ -63,6 +70,41 @@ def cpu_baseline(x, w, seconds):
         "sample": f"{len(times)} passes of the full {rows}x{x.shape[1]} matrix (median), "
                   "four-pass FP32 CPU fastconv (FFT rows, vmmul, inverse FFT rows, 1/N pass), 1 thread",
     }
+
+
+def precision_section(args, x, w, xd, fp32_ms, fp32_rel):
+    """BF16x3 MFMA variant of the same fused fastconv: time per step and
+    error against FP64 next to the FP32 path's."""
+    import numpy as np
+    import torch
+
+    import openvsip_amd as ovx
+    import oracle
+
+    R, N = xd.shape
+    fb = ovx.Fastconv(R, N, precision="bf16x3")
+    wd = torch.from_numpy(w).cuda()
+    yd = torch.empty_like(xd)
+    for _ in range(args.warmup):
+        fb(xd, wd, out=yd)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        fb(xd, wd, out=yd)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    idx = list(range(0, R, max(1, R // 8)))[:8]
+    ref = oracle.fastconv64(x[idx], w)
+    y = yd.cpu().numpy()[idx]
+    return {"kernel": fb.name, "ms_per_step": ms, "rows_per_s": R / (ms / 1e3),
+            "rel_l2_vs_fp64": oracle.rel_l2(y, ref), "max_abs_vs_fp64": float(np.abs(y - ref).max()),
+            "fp32_ms_per_step": fp32_ms, "fp32_rel_l2_vs_fp64": fp32_rel,
+            "note": "radix-16 stages as split-bf16 (hi.hi + hi.lo + lo.hi) MFMA products, FP32 accumulate; "
+                    "spectrum passed per call (not the resident replica)"}
 
 
 def batch_section(args):
@@ -131,6 +173,87 @@ def batch_section(args):
     }
 
 
+def serve_section(args):
+    """Served stream: Poisson arrivals at fixed offered rates of 80% fastconv
+    (N, rows sampled as in the batched workload), 10% forward Fftm (same
+    sampling) and 10% correlation (input 65536, reference 1024, full
+    support); 64 distinct replicas resident; admission by a 64 GB HBM budget.
+    The continuous-batching server (3 slots in flight) against the plain
+    path (one standalone launch per request, 3 in flight).  Requests reuse a
+    pool of 256 device-resident (input, output) pairs per kind; p50/p99 are
+    arrival-to-completion latencies."""
+    import numpy as np
+    import torch
+
+    import openvsip_amd as ovx
+    from openvsip_amd import workloads as wl
+
+    rng = np.random.default_rng(5)
+    sizes = wl.BATCH_SIZES
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    # 64 replicas spread over the sizes
+    reps = {}
+    for i in range(64):
+        n = sizes[i % len(sizes)]
+        reps.setdefault(n, []).append(ovx.Replica(n, wl.lfm_chirp(n, max(1, n // 8 - i)).astype(np.complex64)))
+    cref = ovx.CorrReference("full", wl.cgauss(rng, 1024), 65536)
+    pool = {"fc": [], "fft": [], "corr": []}
+    for kind, count in (("fc", 256), ("fft", 64), ("corr", 32)):
+        for _ in range(count):
+            if kind == "corr":
+                x = torch.randn(65536, dtype=torch.complex64, device="cuda", generator=gen)
+                y = torch.empty(cref.output_size, dtype=torch.complex64, device="cuda")
+                pool[kind].append((x, y, None))
+                continue
+            n = int(rng.choice(sizes))
+            rows = int(rng.integers(16, 257))
+            x = torch.randn(rows, n, dtype=torch.complex64, device="cuda", generator=gen)
+            y = torch.empty_like(x)
+            rep = reps[n][int(rng.integers(len(reps[n])))] if kind == "fc" else None
+            pool[kind].append((x, y, rep))
+
+    def make_trace(count, rate):
+        t = np.cumsum(rng.exponential(1.0 / rate, size=count)) if rate else np.zeros(count)
+        kinds = rng.choice(["fc", "fft", "corr"], size=count, p=[0.8, 0.1, 0.1])
+        reqs = []
+        for ti, k in zip(t, kinds):
+            x, y, rep = pool[k][int(rng.integers(len(pool[k])))]
+            if k == "fc":
+                reqs.append(ovx.fastconv_request(x, rep, y, float(ti)))
+            elif k == "fft":
+                reqs.append(ovx.fftm_request(x, y, float(ti)))
+            else:
+                reqs.append(ovx.corr_request(x, cref, y, "biased", float(ti)))
+        return reqs
+
+    out = {"workload": "Poisson arrivals; 80% fastconv (N in {512..16384,65536,3072,6144,12288}, rows U[16,256]), "
+                       "10% forward Fftm (same sampling), 10% correlation (N=65536, M=1024, full); 64 resident "
+                       "replicas; HBM budget 64 GB; 3 batches in flight",
+           "rates": {}}
+    servers = {"batched": ovx.Server(slots=3, max_jobs=512, hbm_budget_mb=64 * 1024, workspace_mb=2048),
+               "plain": ovx.Server(slots=3, plain=True, hbm_budget_mb=64 * 1024, workspace_mb=2048)}
+    for srv in servers.values():
+        srv.run_trace(make_trace(200, 0))  # warm every size's path and tables
+    for name, srv in servers.items():
+        reqs = make_trace(args.serve_saturation_jobs, 0)
+        lat, mk = srv.run_trace(reqs)
+        out[name + "_saturation_jobs_per_s"] = len(reqs) / mk
+    out["saturation_speedup"] = out["batched_saturation_jobs_per_s"] / out["plain_saturation_jobs_per_s"]
+    for rate in [float(r) for r in args.serve_rates.split(",") if r]:
+        row = {}
+        count = max(50, int(rate * args.serve_seconds))
+        trace = make_trace(count, rate)
+        for name, srv in servers.items():
+            lat, mk = srv.run_trace(trace)
+            row[name] = {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                         "throughput_jobs_per_s": count / mk, "jobs": count}
+        out["rates"][str(int(rate))] = row
+    for srv in servers.values():
+        srv.close()
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -149,12 +272,32 @@ def main():
 
     ovx.init(0)
     R, N = args.rows, args.n
+    if args.copy_probe:
+        src = torch.zeros(R, N, dtype=torch.complex64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.warmup):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        print(json.dumps({"copy_probe": True, "bytes_per_copy": 2 * R * N * 8, "ms_per_copy": ms,
+                          "GBs": 2 * R * N * 8 / (ms * 1e-3) / 1e9}), flush=True)
+        return
     x, w, _, _ = wl.radar(R, N, seed=1)
     xd = torch.from_numpy(x).cuda()
     yd = torch.empty_like(xd)
     # the replica spectrum is resident: transformed once, prepared once in the
     # kernel's layout (the serving path); the timed steps read X, write Y.
     rep = ovx.Replica(N, w, is_spectrum=True)
+    tuned = None
+    if args.variant is None:
+        tuned = ovx.autotune(N, R)  # radix plan / kernel variant for (N, rows), cached
     fc = ovx.Fastconv(R, N, variant=args.variant)
     lib = _lib.lib()
     stream = torch.cuda.current_stream()
@@ -204,14 +347,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"fused fastconv {R}x{N} complex<float>, seed 1, 5 LFM echoes/row, "
                                f"replica chirp 4096 (BT=1024) zero-padded to {N}",
-                   "rows": R, "n": N, "kernel": fc.name},
+                   "rows": R, "n": N, "kernel": fc.name,
+                   "autotuned_variant": None if tuned is None else tuned[0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "gflops": flops / sec / 1e9,
         "parity": {"rel_l2_vs_fp64": rel, "rows_checked": len(idx)},
     }
+    if not args.no_precision and N == 16384:
+        out["precision_bf16x3"] = precision_section(args, x, w, xd, ms, rel)
     if not args.no_batch:
         out["batch"] = batch_section(args)
+    if not args.no_serve:
+        out["serving"] = serve_section(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, w, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -66,6 +66,12 @@ typedef enum {
    cached choice; v >= 1 forces compiled variant v-1, see
    ovx_length_variants). */
 #define OVX_FLAG_VARIANT(v) (((unsigned)(v) & 0xffu) << 8)
+/* Fastconv precision variant (no reference counterpart): the radix-16 DFT
+   stages run on the MFMA cores as BF16 contractions with split (hi + lo)
+   operands, hi.hi + hi.lo + lo.hi, accumulated in FP32.  n = 16384, device
+   pointers, one spectrum vector.  Its error against FP64 is reported beside
+   the FP32 path's; it does not meet the FP32 parity tolerance. */
+#define OVX_FLAG_BF16X3 0x2u
 
 /* Correlation / convolution enums (src/vsip/impl/signal/types.hpp). */
 #define OVX_SUPPORT_FULL 0
@@ -92,6 +98,15 @@ long ovx_max_length(void);
 int ovx_supported_length(long n);
 /* Number of compiled kernel variants for transform length n. */
 int ovx_length_variants(long n);
+/* Autotuner (no reference counterpart; the reference's FFTW backend plans
+   with FFTW_MEASURE, src/ovxx/fftw/fft.cpp:44-52): time every variant of
+   length n on a rows x n fastconv, record the fastest under (n, rows rounded
+   up to a power of two) in the cache file $OVX_TUNE_CACHE (default
+   ~/.cache/ovxhip/tune.txt).  Plans created without an explicit variant use
+   the cached choice. */
+int ovx_autotune_fastconv(long n, long rows, int *best_variant, double *best_us);
+/* Cached variant for (n, rows), or -1. */
+int ovx_tuned_variant(long n, long rows);
 
 /* ---- Fftm (multiple 1-D transforms) ----------------------------------- */
 /* Create a plan for a rows x cols Fftm along `axis` in `direction`, result

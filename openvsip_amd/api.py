@@ -75,6 +75,19 @@ def variants(n):
     return lib().ovx_length_variants(n)
 
 
+def autotune(n, rows):
+    """Time every kernel variant of length n at `rows` rows; cache and return
+    (best variant, microseconds)."""
+    v = ctypes.c_int()
+    us = ctypes.c_double()
+    check(lib().ovx_autotune_fastconv(n, rows, ctypes.byref(v), ctypes.byref(us)))
+    return v.value, us.value
+
+
+def tuned_variant(n, rows):
+    return lib().ovx_tuned_variant(n, rows)
+
+
 def _flags(variant):
     return 0 if variant is None else _lib.FLAG_VARIANT(variant + 1)
 
@@ -124,11 +137,19 @@ class Fft(_Plan):
 class Fastconv(_Plan):
     """Y = scale * IFFT_row( W (.) FFT_row(X) ) in one fused pass."""
 
-    def __init__(self, rows, n, scale=None, variant=None):
+    def __init__(self, rows, n, scale=None, variant=None, precision="fp32"):
+        """precision: "fp32" (FP32 VALU stages, the parity path) or "bf16x3"
+        (radix-16 stages on the MFMA cores as split BF16 products; n = 16384,
+        one spectrum vector)."""
         super().__init__()
         self.rows, self.n = rows, n
         self.scale = 1.0 / n if scale is None else scale
-        check(lib().ovx_fastconv_plan_create(ctypes.byref(self._h), rows, n, self.scale, _flags(variant)))
+        flags = _flags(variant)
+        if precision == "bf16x3":
+            flags |= _lib.FLAG_BF16X3
+        elif precision != "fp32":
+            raise ValueError("precision must be 'fp32' or 'bf16x3'")
+        check(lib().ovx_fastconv_plan_create(ctypes.byref(self._h), rows, n, self.scale, flags))
 
     def __call__(self, x, w, out=None, rows=None):
         torch = _torch()
@@ -345,8 +366,8 @@ class Server:
 
     def __init__(self, slots=3, max_jobs=512, max_tiles=65536, hbm_budget_mb=64 * 1024, workspace_mb=1024,
                  plain=False):
-        cfg = _lib.ServerConfig(1 if plain else slots, 1 if plain else max_jobs, max_tiles, hbm_budget_mb,
-                                workspace_mb, int(bool(plain)))
+        cfg = _lib.ServerConfig(slots, 1 if plain else max_jobs, max_tiles, hbm_budget_mb, workspace_mb,
+                                int(bool(plain)))
         self._h = ctypes.c_void_p()
         check(lib().ovx_server_create(ctypes.byref(self._h), ctypes.byref(cfg)))
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/ovxhip.h"
+#include "ovx_bf16.hpp"
 #include "ovx_fourstep.hpp"
 #include "ovx_hier.hpp"
 #include "ovx_internal.hpp"
@@ -220,17 +221,29 @@ int LineFFT::transform(int dir, const float2 *in, long ld_in, float2 *out, long 
     return OVX_OK;
   }
   if (kind == FOURSTEP) {
-    if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step transform");
     int st = work.alloc((size_t)lines * n * 8);
     if (st) return st;
+    return transform_ws(dir, in, ld_in, out, ld_out, lines, scale, (float2 *)work.p, s);
+  }
+  return fail(OVX_ERR_UNSUPPORTED, "no transform path");
+}
+
+int LineFFT::transform_ws(int dir, const float2 *in, long ld_in, float2 *out, long ld_out, long lines, float scale,
+                          float2 *ws, hipStream_t s) const
+{
+  if (kind != FOURSTEP) return transform(dir, in, ld_in, out, ld_out, lines, scale, s);
+  if (lines <= 0) return OVX_OK;
+  if (!tables_ready() || !ws) return fail(OVX_ERR_STATE, "four-step transform without tables or workspace");
+  {
+    if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step transform");
     FsArgs a = fs_args(*this);
     a.in = in;
     a.ld_in = ld_in;
-    a.out = (float2 *)work.p;
+    a.out = ws;
     a.ld_out = n;
     hipError_t e = launch_fs_col(dir < 0 ? 0 : 2, a, lines, s);
     if (e != hipSuccess) return hip_fail(e, "four-step column pass");
-    a.in = (const float2 *)work.p;
+    a.in = ws;
     a.ld_in = n;
     a.out = out;
     a.ld_out = ld_out;
@@ -239,7 +252,6 @@ int LineFFT::transform(int dir, const float2 *in, long ld_in, float2 *out, long 
     if (e != hipSuccess) return hip_fail(e, "four-step row pass");
     return OVX_OK;
   }
-  return fail(OVX_ERR_UNSUPPORTED, "no transform path");
 }
 
 int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, float2 *y, long ldy, long lines,
@@ -417,10 +429,21 @@ extern "C" int ovx_length_variants(long n)
   return LineFFT::variants(n);
 }
 
-static int variant_of(unsigned flags)
+namespace ovx {
+int tuned_variant(long n, long rows);
+}
+
+// Explicit variant from the flags, else the autotuner's cached choice for
+// (n, rows), else the default (variant 0).
+static int variant_of(unsigned flags, long n = 0, long rows = 0)
 {
   int v = (int)((flags >> 8) & 0xffu);
-  return v ? v - 1 : 0;
+  if (v) return v - 1;
+  if (n > 0 && rows > 0) {
+    int t = tuned_variant(n, rows);
+    if (t >= 0 && t < LineFFT::variants(n)) return t;
+  }
+  return 0;
 }
 
 extern "C" int ovx_supported_length(long n)
@@ -450,7 +473,7 @@ extern "C" int ovx_fftm_plan_create(ovx_plan *plan, long rows, long cols, int ax
   p->flags = flags;
   long n = axis == OVX_ROW ? cols : rows;
   if (n > 0) {
-    int st = p->line.init(n, variant_of(flags));
+    int st = p->line.init(n, variant_of(flags, n, axis == OVX_ROW ? rows : cols));
     if (st) return st;
   }
   *plan = p.release();
@@ -540,7 +563,20 @@ extern "C" int ovx_fastconv_plan_create(ovx_plan *plan, long rows, long n, float
   p->cols = n;
   p->scale = scale;
   p->flags = flags;
-  int st = p->line.init(n, variant_of(flags));
+  if (flags & OVX_FLAG_BF16X3) {
+    if (n != bf16x3_length() || (flags & OVX_FLAG_HOST_POINTERS))
+      return fail(OVX_ERR_UNSUPPORTED, "the BF16x3 MFMA variant runs n = %d on device pointers", bf16x3_length());
+    p->bf16x3 = true;
+    // its stage plan and twiddles are those of the register-Stockham
+    // configuration with 16 values per thread
+    int st = p->line.init(n, hier_supported(n) ? 1 : 0);
+    if (st) return st;
+    if (p->line.kind != LineFFT::LDS || p->line.rc.p != 16 || p->line.rc.plan.twtotal != bf16x3_twiddle_count())
+      return fail(OVX_ERR_STATE, "BF16x3 variant: unexpected stage plan");
+    *plan = p.release();
+    return OVX_OK;
+  }
+  int st = p->line.init(n, variant_of(flags, n, rows));
   if (st) return st;
   *plan = p.release();
   return OVX_OK;
@@ -558,6 +594,13 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
   if (ldw != 0 && rows > 1 && ldw < n) return fail(OVX_ERR_INVALID, "coefficient row stride shorter than n");
   if (rows == 0) return OVX_OK;
   hipStream_t s = (hipStream_t)stream;
+  if (p->bf16x3) {
+    if (ldw != 0) return fail(OVX_ERR_UNSUPPORTED, "BF16x3 variant takes one spectrum vector");
+    hipError_t e = launch_bf16x3_fastconv((const float2 *)x, ldx, (const float2 *)w, (float2 *)y, ldy, rows,
+                                          (const float2 *)p->line.tw.p, p->scale, s);
+    if (e != hipSuccess) return hip_fail(e, "bf16x3 fastconv launch");
+    return OVX_OK;
+  }
   if (p->flags & OVX_FLAG_HOST_POINTERS) {
     // stage x and w together, compute into stage_out
     size_t xb = ((size_t)(rows - 1) * ldx + n) * 8;
@@ -581,7 +624,7 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
 namespace {
 struct Replica {
   long n = 0;
-  DevBuf spec, prep;
+  DevBuf spec, prep, nat; // nat: natural order, 1/n folded in (register-Stockham paths)
   int prep_kind = 0, prep_variant = -1;
 };
 std::mutex g_rep_mu;
@@ -620,6 +663,14 @@ extern "C" int ovx_replica_create(long n, const void *replica, long m, int is_sp
   if (e != hipSuccess) return hip_fail(e, "replica preparation");
   r->prep_kind = f.kind;
   r->prep_variant = f.variant;
+  if (f.kind != LineFFT::LDS && f.kind != LineFFT::FOURSTEP) {
+    // a tuned plan may run this length on the register-Stockham path, which
+    // reads the spectrum in natural order
+    if ((st = r->nat.alloc((size_t)n * 8))) return st;
+    hipError_t e2 = launch_scale_copy((const float2 *)r->spec.p, (float2 *)r->nat.p, n, 1.0f / (float)n, nullptr);
+    if (e2 == hipSuccess) e2 = hipStreamSynchronize(nullptr);
+    if (e2 != hipSuccess) return hip_fail(e2, "replica preparation");
+  }
   std::lock_guard<std::mutex> g(g_rep_mu);
   *id = g_next_rep++;
   g_replicas[*id] = std::move(r);
@@ -654,10 +705,15 @@ int replica_prepared(int id, const float2 **prep, long *n, const LineFFT *f)
   if (it == g_replicas.end()) return fail(OVX_ERR_INVALID, "unknown replica id %d", id);
   // register-Stockham paths read the spectrum in natural order whatever
   // their variant; the other paths need the layout of their own variant
-  bool ok = it->second->prep_kind == f->kind &&
-            (f->kind == LineFFT::LDS || it->second->prep_variant == f->variant);
+  const Replica &r = *it->second;
+  if (f->kind == LineFFT::LDS && r.prep_kind != LineFFT::LDS && r.nat.p) {
+    *prep = (const float2 *)r.nat.p;
+    *n = r.n;
+    return OVX_OK;
+  }
+  bool ok = r.prep_kind == f->kind && (f->kind == LineFFT::LDS || r.prep_variant == f->variant);
   if (!ok) return fail(OVX_ERR_STATE, "replica %d is not prepared for this kernel path", id);
-  *prep = (const float2 *)it->second->prep.p;
+  *prep = (const float2 *)r.prep.p;
   *n = it->second->n;
   return OVX_OK;
 }
@@ -674,21 +730,19 @@ extern "C" int ovx_fastconv_exec_replica(ovx_plan p, const void *x, long ldx, in
   if (rows < 0 || rows > p->rows) return fail(OVX_ERR_INVALID, "rows %ld outside the plan's %ld", rows, p->rows);
   long n = p->cols;
   if (rows > 1 && (ldx < n || ldy < n)) return fail(OVX_ERR_INVALID, "row stride shorter than n");
-  const float2 *spec, *prep;
-  int pk, pv;
+  const float2 *spec, *prep = nullptr;
+  long rn = 0;
   {
     std::lock_guard<std::mutex> g(g_rep_mu);
     auto it = g_replicas.find(replica);
     if (it == g_replicas.end()) return fail(OVX_ERR_INVALID, "unknown replica id %d", replica);
     if (it->second->n != n) return fail(OVX_ERR_INVALID, "replica length %ld != plan length %ld", it->second->n, n);
     spec = (const float2 *)it->second->spec.p;
-    prep = (const float2 *)it->second->prep.p;
-    pk = it->second->prep_kind;
-    pv = it->second->prep_variant;
   }
   hipStream_t s = (hipStream_t)stream;
   if (rows == 0) return OVX_OK;
-  if (pk == p->line.kind && (pk == LineFFT::LDS || pv == p->line.variant) && p->scale == 1.0f / (float)n)
+  if (p->bf16x3) return ovx_fastconv_exec(p, x, ldx, spec, 0, y, ldy, rows, stream);
+  if (p->scale == 1.0f / (float)n && replica_prepared(replica, &prep, &rn, &p->line) == OVX_OK)
     return p->line.fastconv_prepared((const float2 *)x, ldx, prep, (float2 *)y, ldy, rows, s);
   return p->line.fastconv((const float2 *)x, ldx, spec, 0, (float2 *)y, ldy, rows, p->scale, s);
 }

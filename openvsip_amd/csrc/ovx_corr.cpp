@@ -99,11 +99,18 @@ ovx_plan_s::~ovx_plan_s() { delete corr; }
 const char *ovx_plan_s::name() const
 {
   switch (kind) {
-  case PLAN_FFTM: return line.kind == LineFFT::LDS ? "fftm:lds-resident" : line.kind == LineFFT::FOURSTEP ? "fftm:four-step" : "fftm:none";
-  case PLAN_FFT: return line.kind == LineFFT::LDS ? "fft:lds-resident" : line.kind == LineFFT::FOURSTEP ? "fft:four-step" : "fft:none";
+  case PLAN_FFTM: return line.kind == LineFFT::LDS ? "fftm:lds-resident" : line.kind == LineFFT::FOURSTEP ? "fftm:four-step" : line.kind == LineFFT::HIER ? "fftm:hierarchical-lds" : "fftm:none";
+  case PLAN_FFT:
+    return line.kind == LineFFT::LDS        ? "fft:lds-resident"
+           : line.kind == LineFFT::FOURSTEP ? "fft:four-step"
+           : line.kind == LineFFT::HIER     ? "fft:hierarchical-lds"
+                                            : "fft:none";
   case PLAN_FASTCONV:
-    return line.kind == LineFFT::LDS ? "fastconv:fused-lds-resident"
-                                     : line.kind == LineFFT::FOURSTEP ? "fastconv:four-step-3pass" : "fastconv:none";
+    if (bf16x3) return "fastconv:bf16x3-mfma";
+    return line.kind == LineFFT::LDS        ? "fastconv:fused-lds-resident"
+           : line.kind == LineFFT::FOURSTEP ? "fastconv:four-step-3pass"
+           : line.kind == LineFFT::HIER     ? "fastconv:fused-hierarchical-lds"
+                                            : "fastconv:none";
   case PLAN_CORR: return "corr:overlap-save";
   case PLAN_CONV: return "conv:overlap-save";
   default: return "unknown";

@@ -48,6 +48,10 @@ struct LineFFT {
   // lines x n at line strides ld_in / ld_out; dir = -1 forward, +1 inverse
   int transform(int dir, const float2 *in, long ld_in, float2 *out, long ld_out, long lines, float scale,
                 hipStream_t s) const;
+  // same, with a caller-provided four-step workspace of lines * n elements
+  // (so that concurrent callers of one path do not share `work`)
+  int transform_ws(int dir, const float2 *in, long ld_in, float2 *out, long ld_out, long lines, float scale,
+                   float2 *ws, hipStream_t s) const;
   int fastconv(const float2 *x, long ldx, const float2 *w, long ldw, float2 *y, long ldy, long lines, float scale,
                hipStream_t s) const;
   // Spectrum layout the fastconv kernels of this path read: wp = scale * w
@@ -77,6 +81,7 @@ struct ovx_plan_s {
   ovx::LineFFT line;
   ovx::DevBuf stage_in, stage_out, stage_w;
   ovx::CorrState *corr = nullptr;
+  bool bf16x3 = false; // fastconv radix-16 stages on MFMA (precision variant)
 
   ~ovx_plan_s();
   const char *name() const;

@@ -201,7 +201,7 @@ int Server::init(const ovx_server_config &c)
   cfg = c;
   if (cfg.slots < 1) cfg.slots = 1;
   if (cfg.max_jobs < 1) cfg.max_jobs = 1;
-  if (cfg.plain) cfg.max_jobs = 1;
+  if (cfg.plain) cfg.max_jobs = 1; // one request per launch, `slots` requests in flight
   budget = (size_t)std::max(cfg.hbm_budget_mb, 1l) << 20;
   max_tiles = cfg.max_tiles > 0 ? cfg.max_tiles : 65536;
   (void)hipGetDevice(&device);
@@ -325,7 +325,8 @@ int Server::launch_plain(Slot &s, Req &q)
     if ((st = replica_prepared(r.ref, &prep, &rn, f))) return st;
     return f->fastconv_prepared((const float2 *)r.x, r.ldx, prep, (float2 *)r.y, r.ldy, r.rows, s.stream);
   }
-  return f->transform(-1, (const float2 *)r.x, r.ldx, (float2 *)r.y, r.ldy, r.rows, 1.0f, s.stream);
+  return f->transform_ws(-1, (const float2 *)r.x, r.ldx, (float2 *)r.y, r.ldy, r.rows, 1.0f, (float2 *)s.ws.p,
+                         s.stream);
 }
 
 void Server::scheduler()

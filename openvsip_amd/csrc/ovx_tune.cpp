@@ -1,0 +1,152 @@
+// ovx_tune.cpp -- autotuner for the kernel variant of a (transform length,
+// rows) pair, with a persistent cache.
+//
+// Every compiled variant of a length (hierarchical, register-Stockham with
+// different values per thread) gives the same result to rounding; which is
+// fastest depends on the length and on how many rows share the GPU.  The
+// tuner times each variant's fused fastconv on scratch data of the requested
+// shape and records the winner under (n, rows rounded up to a power of two).
+// Plans created with the default variant flag consult the cache.  The cache
+// is a text file, one "n rows_bucket variant microseconds" line per entry,
+// at $OVX_TUNE_CACHE (default ~/.cache/ovxhip/tune.txt); it is the only state
+// the library persists.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <sys/stat.h>
+#include <utility>
+
+#include "ovx_plan.hpp"
+
+namespace ovx {
+
+static std::mutex g_tune_mu;
+static std::map<std::pair<long, long>, std::pair<int, double>> g_tune;
+static bool g_tune_loaded = false;
+
+static long rows_bucket(long rows)
+{
+  long b = 1;
+  while (b < rows) b <<= 1;
+  return b;
+}
+
+static std::string cache_path()
+{
+  const char *p = getenv("OVX_TUNE_CACHE");
+  if (p && *p) return p;
+  const char *home = getenv("HOME");
+  return std::string(home ? home : "/tmp") + "/.cache/ovxhip/tune.txt";
+}
+
+static void load_locked()
+{
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  FILE *f = fopen(cache_path().c_str(), "r");
+  if (!f) return;
+  long n, rb;
+  int v;
+  double us;
+  while (fscanf(f, "%ld %ld %d %lf", &n, &rb, &v, &us) == 4) g_tune[{ n, rb }] = { v, us };
+  fclose(f);
+}
+
+static void save_locked()
+{
+  std::string path = cache_path();
+  std::string dir = path.substr(0, path.find_last_of('/'));
+  // create the directory chain (ignore failures: the cache is optional)
+  for (size_t i = 1; i <= dir.size(); ++i)
+    if (i == dir.size() || dir[i] == '/') mkdir(dir.substr(0, i).c_str(), 0755);
+  std::string tmp = path + ".tmp";
+  FILE *f = fopen(tmp.c_str(), "w");
+  if (!f) return;
+  for (auto &e : g_tune) fprintf(f, "%ld %ld %d %.3f\n", e.first.first, e.first.second, e.second.first, e.second.second);
+  fclose(f);
+  rename(tmp.c_str(), path.c_str());
+}
+
+// Variant cached for (n, rows), or -1.
+int tuned_variant(long n, long rows)
+{
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  load_locked();
+  auto it = g_tune.find({ n, rows_bucket(rows) });
+  return it == g_tune.end() ? -1 : it->second.first;
+}
+
+static int time_variant(long n, long rows, int variant, float2 *x, float2 *w, float2 *y, hipStream_t s,
+                        double *us)
+{
+  LineFFT f;
+  int st = f.init(n, variant);
+  if (st) return st;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(OVX_ERR_DEVICE, "event create");
+  for (int i = 0; i < 3 && !st; ++i) st = f.fastconv(x, n, w, 0, y, n, rows, 1.0f / (float)n, s);
+  const int iters = 20;
+  hipError_t e = hipEventRecord(e0, s);
+  for (int i = 0; i < iters && !st; ++i) st = f.fastconv(x, n, w, 0, y, n, rows, 1.0f / (float)n, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  if (e != hipSuccess) return hip_fail(e, "tuning timing");
+  *us = ms * 1e3 / iters;
+  return OVX_OK;
+}
+
+} // namespace ovx
+
+using namespace ovx;
+
+extern "C" int ovx_autotune_fastconv(long n, long rows, int *best_variant, double *best_us)
+{
+  if (n < 1 || rows < 1) return fail(OVX_ERR_INVALID, "bad shape");
+  int nv = LineFFT::variants(n);
+  if (nv < 1) return fail(OVX_ERR_UNSUPPORTED, "length %ld not supported", n);
+  DevBuf x, w, y;
+  size_t bytes = (size_t)rows * n * 8;
+  int st;
+  if ((st = x.alloc(bytes)) || (st = y.alloc(bytes)) || (st = w.alloc((size_t)n * 8))) return st;
+  hipError_t e = hipMemset(x.p, 0, bytes);
+  if (e == hipSuccess) e = hipMemset(w.p, 0, (size_t)n * 8);
+  if (e != hipSuccess) return hip_fail(e, "tuning buffers");
+  hipStream_t s;
+  if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "tuning stream");
+  int best = 0;
+  double best_t = 1e30;
+  for (int v = 0; v < nv; ++v) {
+    double us;
+    st = time_variant(n, rows, v, (float2 *)x.p, (float2 *)w.p, (float2 *)y.p, s, &us);
+    if (st == OVX_ERR_UNSUPPORTED) continue; // shape outside this variant's limits
+    if (st) break;
+    if (us < best_t) {
+      best_t = us;
+      best = v;
+    }
+  }
+  (void)hipStreamDestroy(s);
+  if (st && st != OVX_ERR_UNSUPPORTED) return st;
+  if (best_t >= 1e30) return fail(OVX_ERR_UNSUPPORTED, "no variant ran for n=%ld rows=%ld", n, rows);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    load_locked();
+    g_tune[{ n, rows_bucket(rows) }] = { best, best_t };
+    save_locked();
+  }
+  if (best_variant) *best_variant = best;
+  if (best_us) *best_us = best_t;
+  return OVX_OK;
+}
+
+extern "C" int ovx_tuned_variant(long n, long rows) { return tuned_variant(n, rows); }

@@ -184,6 +184,40 @@ def test_fastconv_matched_filter_peaks(ovx):
             assert abs(peak - a * energy) < 0.2 * energy, (r, d, a, peak)
 
 
+# Error bound of the BF16x3 variant against FP64.  Each operand is split as
+# hi + lo with |x - hi - lo| <= 2^-16 |x| (bf16 keeps 8 significant bits) and
+# the dropped lo.lo product is of the same order, so every radix-16 stage adds
+# about 1e-5 relative error (RMS); six such stages plus the FP32 steps stay
+# below this bound.  It is the precision variant's stated bound, looser than
+# the FP32 path's parity tolerance.
+BF16X3_REL_L2 = 1e-4
+
+
+def test_fastconv_bf16x3_mfma_vs_fp64(ovx):
+    x, w, delays, amps = wl.radar(64, 16384, seed=7)
+    ref = o.fastconv64(x, w)
+    fc32 = ovx.Fastconv(64, 16384)
+    fcb = ovx.Fastconv(64, 16384, precision="bf16x3")
+    assert fcb.name == "fastconv:bf16x3-mfma"
+    xd, wd = dev(x), dev(w)
+    y32 = host(fc32(xd, wd))
+    yb = host(fcb(xd, wd))
+    e32, eb = o.rel_l2(y32, ref), o.rel_l2(yb, ref)
+    say("bf16x3 rel_l2", eb, "max_abs", np.abs(yb - ref).max(), "| fp32 rel_l2", e32,
+        "max_abs", np.abs(y32 - ref).max())
+    assert e32 < REL_L2
+    assert eb < BF16X3_REL_L2
+    # ragged rows, strided input and in-place output
+    xs = np.zeros((5, 16384 + 64), np.complex64)
+    xs[:, :16384] = x[:5]
+    xsd = dev(xs)
+    fcb(xsd[:, :16384], wd, out=xsd[:, :16384], rows=5)
+    assert o.rel_l2(host(xsd)[:, :16384], ref[:5]) < BF16X3_REL_L2
+    # a spectrum matrix is refused
+    with pytest.raises(ovx.OvxUnsupported):
+        fcb(xd, dev(np.repeat(w[None, :], 64, axis=0)))
+
+
 def test_fastconv_coefficient_matrix_and_in_place(ovx):
     rows, n = 33, 4096
     rng = np.random.default_rng(12)
@@ -331,4 +365,19 @@ def test_fastconv_resident_replica(ovx, n):
     fc = ovx.Fastconv(rows, n)
     y = host(fc.with_replica(dev(x), rep))
     w = wl.replica_spectrum(chirp.astype(np.complex128), n)
+    assert o.rel_l2(y, o.fastconv64(x, w)) < REL_L2
+
+
+def test_autotune_records_and_plans_use_it(ovx, tmp_path, monkeypatch):
+    monkeypatch.setenv("OVX_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    v, us = ovx.autotune(4096, 300)
+    assert 0 <= v < ovx.variants(4096) and us > 0
+    # rows 300 -> bucket 512; 400 shares it
+    assert ovx.tuned_variant(4096, 400) == v
+    assert (tmp_path / "tune.txt").exists()
+    rng = np.random.default_rng(31)
+    x = cg(rng, (300, 4096))
+    w = cg(rng, 4096)
+    fc = ovx.Fastconv(300, 4096)
+    y = host(fc(dev(x), dev(w)))
     assert o.rel_l2(y, o.fastconv64(x, w)) < REL_L2
