@@ -5,13 +5,15 @@
 // different values per thread) gives the same result to rounding; which is
 // fastest depends on the length and on how many rows share the GPU.  The
 // tuner times each variant's fused fastconv on scratch data of the requested
-// shape and records the winner under (n, rows rounded up to a power of two).
+// shape (three interleaved rounds, best round per variant) and records the
+// winner under (n, rows rounded up to a power of two).
 // Plans created with the default variant flag consult the cache.  The cache
 // is a text file, one "n rows_bucket variant microseconds" line per entry,
 // at $OVX_TUNE_CACHE (default ~/.cache/ovxhip/tune.txt); it is the only state
 // the library persists.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -19,6 +21,7 @@
 #include <string>
 #include <sys/stat.h>
 #include <utility>
+#include <vector>
 
 #include "ovx_plan.hpp"
 
@@ -123,18 +126,25 @@ extern "C" int ovx_autotune_fastconv(long n, long rows, int *best_variant, doubl
   if (e != hipSuccess) return hip_fail(e, "tuning buffers");
   hipStream_t s;
   if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "tuning stream");
-  int best = 0;
-  double best_t = 1e30;
-  for (int v = 0; v < nv; ++v) {
-    double us;
-    st = time_variant(n, rows, v, (float2 *)x.p, (float2 *)w.p, (float2 *)y.p, s, &us);
-    if (st == OVX_ERR_UNSUPPORTED) continue; // shape outside this variant's limits
-    if (st) break;
-    if (us < best_t) {
-      best_t = us;
-      best = v;
+  // three interleaved rounds, the fastest round of each variant counts
+  // (robust to clock ramp-up and to a busy device during one round)
+  std::vector<double> t_min(nv, 1e30);
+  for (int round = 0; round < 3 && (!st || st == OVX_ERR_UNSUPPORTED); ++round) {
+    for (int v = 0; v < nv; ++v) {
+      double us;
+      st = time_variant(n, rows, v, (float2 *)x.p, (float2 *)w.p, (float2 *)y.p, s, &us);
+      if (st == OVX_ERR_UNSUPPORTED) continue; // shape outside this variant's limits
+      if (st) break;
+      t_min[v] = std::min(t_min[v], us);
     }
   }
+  int best = 0;
+  double best_t = 1e30;
+  for (int v = 0; v < nv; ++v)
+    if (t_min[v] < best_t) {
+      best_t = t_min[v];
+      best = v;
+    }
   (void)hipStreamDestroy(s);
   if (st && st != OVX_ERR_UNSUPPORTED) return st;
   if (best_t >= 1e30) return fail(OVX_ERR_UNSUPPORTED, "no variant ran for n=%ld rows=%ld", n, rows);
