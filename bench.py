@@ -48,28 +48,67 @@ def parse():
     return ap.parse_args()
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(x, w, seconds):
     """Single-core FP32 four-pass fastconv over the same matrix, repeated
-    until `seconds` of CPU work; median pass time -> rows/s."""
+    until `seconds` of CPU work; median pass time -> rows/s.  Also the same
+    code with the rows split over the host's cores (not the reference's
+    behaviour: its fftm backend loops rows on one thread)."""
     import numpy as np
     import oracle
     rows = x.shape[0]
     oracle.fastconv_port(x[:2], w)  # warm the code path
-    times = []
-    t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        oracle.fastconv_port(x, w, threads=1)
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
+
+    def median_pass(threads, budget):
+        times = []
+        t_end = time.perf_counter() + budget
+        while time.perf_counter() < t_end or len(times) < 5:
+            t0 = time.perf_counter()
+            oracle.fastconv_port(x, w, threads=threads)
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), len(times)
+
+    med, count = median_pass(1, seconds)
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    med_all, count_all = median_pass(cores, max(1.0, seconds / 4))
     return {
         "value": rows / med,
         "unit": "rows/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{len(times)} passes of the full {rows}x{x.shape[1]} matrix (median), "
+        "sample": f"{count} passes of the full {rows}x{x.shape[1]} matrix (median), "
                   "four-pass FP32 CPU fastconv (FFT rows, vmmul, inverse FFT rows, 1/N pass), 1 thread",
+        "cpu_model": _cpu_model(),
+        "all_cores": {"value": rows / med_all, "unit": "rows/s", "cores": cores,
+                      "sample": f"{count_all} passes (median), rows split over {cores} threads; "
+                                "not the reference's behaviour"},
     }
+
+
+def measured_traffic(plan_name, rows, n):
+    """HBM bytes per launch from the committed rocprof counter summary of this
+    kernel at this shape (FETCH_SIZE and WRITE_SIZE, each calibrated against
+    a device copy), or None when none was recorded."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"k_hier_fastconv_{n}x{rows}_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except OSError:
+        return None
+    if rec.get("plan_name") != plan_name or rec.get("rows") != rows or rec.get("n") != n:
+        return None
+    return rec["traffic_bytes"]
 
 
 def precision_section(args, x, w, xd, fp32_ms, fp32_rel):
@@ -358,7 +397,7 @@ def main():
                    "rows": R, "n": N, "kernel": fc.name,
                    "autotuned_variant": None if tuned is None else tuned[0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(fc.name, R, N)},
         "gflops": flops / sec / 1e9,
         "parity": {"rel_l2_vs_fp64": rel, "rows_checked": len(idx)},
     }

@@ -18,8 +18,9 @@
 
 namespace ovx {
 
-// LDS of the largest tile (1024 threads): hierarchical rows 17408 float2,
-// four-step groups 4 x 16 x 273.
+// LDS of the largest tile (1024 threads): 64 register-Stockham rows of 256
+// (17408 float2), four-step groups 4 x 16 x 273; every tile function checks
+// its need against it.
 constexpr int kBatchLds = 4 * 16 * 273 > 17408 ? 4 * 16 * 273 : 17408;
 
 // The job descriptor of the current tile lives in LDS (copied once per tile
@@ -30,6 +31,7 @@ template <int N>
 __device__ __noinline__ void tile_hier_fastconv(lds_job &j, int row0, lds_ptr lds)
 {
   using H = Hier<N>;
+  static_assert((kBatchThreads / H::TR) * H::LROW <= kBatchLds, "tile LDS");
   HierArgs a{};
   a.x = j.x;
   a.ldx = j.ldx;
@@ -51,6 +53,7 @@ template <int N>
 __device__ __noinline__ void tile_hier_fwd(lds_job &j, int row0, lds_ptr lds)
 {
   using H = Hier<N>;
+  static_assert((kBatchThreads / H::TR) * H::LROW <= kBatchLds, "tile LDS");
   HierArgs a{};
   a.x = j.x;
   a.ldx = j.ldx;
@@ -69,6 +72,7 @@ template <int N, int P>
 __device__ __noinline__ void tile_row_fastconv(lds_job &j, int row0, lds_ptr lds)
 {
   using E = Engine<N, P, (N / P <= 64)>;
+  static_assert((kBatchThreads / E::T) * E::LROW <= kBatchLds, "tile LDS");
   const int lrow = (int)threadIdx.x / E::T;
   const int t = (int)threadIdx.x % E::T;
   row_fastconv_body<N, P>(j.x, j.ldx, j.w, 0, j.y, j.ldy, j.rows, j.tw0, 1.0f, row0, lrow, t,
@@ -79,6 +83,7 @@ template <int N, int P>
 __device__ __noinline__ void tile_row_fwd(lds_job &j, int row0, lds_ptr lds)
 {
   using E = Engine<N, P, (N / P <= 64)>;
+  static_assert((kBatchThreads / E::T) * E::LROW <= kBatchLds, "tile LDS");
   const int lrow = (int)threadIdx.x / E::T;
   const int t = (int)threadIdx.x % E::T;
   row_fft_body<N, P, -1>(j.x, j.ldx, j.y, j.ldy, j.rows, j.tw0, j.scale, row0, lrow, t, lds + lrow * E::LROW);
@@ -108,6 +113,7 @@ __device__ __noinline__ void tile_ols(lds_job &j, int seg0, lds_ptr lds)
   a.out = j.ols.out;
   constexpr int T = L / 16;
   static_assert(T <= kBatchThreads, "segment too long");
+  static_assert((kBatchThreads / T) * Engine<L, 16>::LROW <= kBatchLds, "tile LDS");
   const int t = (int)threadIdx.x % T;
   const int copy = (int)threadIdx.x / T;
   const long seg = (long)seg0 + copy;
@@ -129,6 +135,7 @@ __device__ __forceinline__ FsArgs fs_args_of(lds_job &j)
   a.tlo = j.tlo;
   a.thi = j.thi;
   a.sbits = j.sbits;
+  a.tfull = j.tfull;
   a.wp = j.w;
   a.ldwp = 0;
   a.scale = 1.0f;
@@ -140,6 +147,7 @@ template <int L, int PASS>
 __device__ __noinline__ void tile_fs(lds_job &j, int row, int group0, lds_ptr lds)
 {
   using F = FsCfg<L>;
+  static_assert((kBatchThreads / F::NT) * F::LDS <= kBatchLds, "tile LDS");
   constexpr int G = kBatchThreads / F::NT; // groups per tile
   const int g = (int)threadIdx.x / F::NT;
   const int tid = (int)threadIdx.x % F::NT;
@@ -167,6 +175,7 @@ template <int L, int PASS>
 __device__ __noinline__ void tile_fs_fft(lds_job &j, int row, int group0, lds_ptr lds)
 {
   using F = FsCfg<L>;
+  static_assert((kBatchThreads / F::NT) * F::LDS <= kBatchLds, "tile LDS");
   const int g = (int)threadIdx.x / F::NT;
   const int tid = (int)threadIdx.x % F::NT;
   FsArgs a = fs_args_of(j);

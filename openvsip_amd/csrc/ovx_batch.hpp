@@ -35,6 +35,7 @@ struct BatchJob {
   const float2 *tw1; // HIER: 1024-point wave table; FS: n2 line table
   const float2 *tlo, *thi;
   int sbits;
+  const float2 *tfull; // FS: full twiddle table or null
   int n1, n2;
   float scale;       // forward Fftm scale
   float2 *ws;        // four-step Fftm workspace (rows x n)

@@ -102,6 +102,7 @@ int make_batch_job(const LineFFT &f, long rows, const float2 *x, long ldx, float
     j.tlo = (const float2 *)f.tlo.p;
     j.thi = (const float2 *)f.thi.p;
     j.sbits = f.sbits;
+    j.tfull = (const float2 *)f.tfull.p;
     j.n1 = (int)f.n1;
     j.n2 = (int)f.n2;
   }

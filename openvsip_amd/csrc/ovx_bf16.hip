@@ -2,53 +2,89 @@
 // the radix-16 DFT stages run as dense contractions on the MFMA cores with
 // split-precision BF16 products accumulated in FP32.
 //
-// A radix-16 DFT of 16 butterflies at once is the real product
-//   [Yr; Yi] (32 x 16) = [[Fr, -Fi], [Fi, Fr]] (32 x 32) . [Xr; Xi] (32 x 16)
-// with F[q][r] = exp(DIR 2 pi i q r / 16).  Every operand is split into a
-// bf16 high part and a bf16 low part (x = hi + lo, |x - hi - lo| <= 2^-16 |x|, bf16 keeping 8
-// significant bits)
-// and the product is hi.hi + hi.lo + lo.hi, three
-// v_mfma_f32_16x16x32_bf16 per half (Yr rows, Yi rows), accumulated in FP32.
-// The DFT matrix entries are the FP32-rounded values of cos and sin of
-// multiples of 2 pi/16.  Twiddles, the last
-// (radix-4) stage and the spectrum multiply stay in FP32 on the VALU.  Data
-// between stages live in LDS in planar form (real and imaginary arrays).
-// Stage structure (16, 16, 16, 4) and twiddle table are those of the
-// register-Stockham path Engine<16384, 16>.
+// Stage structure (radices 16, 16, 16, 4) and twiddle table are those of the
+// register-Stockham path Engine<16384, 16>: stage s, butterfly j < 1024,
+// k = j mod Ns, reads x[j + 1024 r] (r < 16), multiplies by W_{16 Ns}^{r k},
+// takes the 16-point DFT and writes output q to
+// (j / Ns) 16 Ns + k + q Ns.
+//
+// One MFMA group is 16 butterflies (the N = 16 columns).  Its DFT is the real
+// product Y (32 x 16) = A (32 x 32) . B (32 x 16): the K index runs over
+// (r, re/im) pairs, B[(r, re)] = Re x_r, B[(r, im)] = Im x_r, and the top and
+// bottom halves of A give Re y_q and Im y_q:
+//   A_top[q][(r, re)] = Re c_qr,  A_top[q][(r, im)] = -Im c_qr,
+//   A_bot[q][(r, re)] = Im c_qr,  A_bot[q][(r, im)] =  Re c_qr,
+// with c_qr the coefficient of x_r in y_q.  For the first stage c_qr is the
+// DFT-16 entry exp(DIR 2 pi i q r / 16); in the second stage (Ns = 16) the 16
+// butterflies of a group share one k, so the twiddle folds into A:
+// c_qr = exp(DIR 2 pi i r (16 q + k) / 256), one wave per k; in the third
+// stage (Ns = 256) the twiddles are applied to B on the VALU.
+//
+// Every operand is split into a bf16 high part and a bf16 low part
+// (x = hi + lo, |x - hi - lo| <= 2^-16 |x|, bf16 keeping 8 significant bits)
+// and the product is lo.hi + hi.lo + hi.hi, three v_mfma_f32_16x16x32_bf16
+// per half, accumulated in FP32.  The last (radix-4) stage, the spectrum
+// multiply and the third stage's twiddles stay in FP32 on the VALU.  The row
+// lives in LDS (interleaved complex, swizzled so that the strides of every
+// stage's accesses hit distinct banks).
 //
 // MFMA operand layout (16x16x32 bf16): lane l holds A[l % 16][8 (l/16) + i]
 // and B[8 (l/16) + i][l % 16], i < 8, and D[4 (l/16) + i][l % 16], i < 4.
+// Lane (c = l % 16, h = l / 16) therefore holds inputs r = 4h .. 4h+3 of
+// butterfly column c, and outputs q = 4h .. 4h+3 of that column.
 #include "ovx_bf16.hpp"
 #include "ovx_internal.hpp"
 
 namespace ovx {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 constexpr int kN = 16384;
-constexpr int kT = 1024;             // threads
-constexpr int kGroups = kN / 16 / 16; // groups of 16 butterflies per radix-16 stage
-constexpr int kGpw = kGroups / 16;    // groups per wave
-using E = Engine<kN, 16>;             // stage plan (16, 16, 16, 4) and twiddles
-constexpr int pad(int pos) { return pos + (pos >> 4); }
-constexpr int kLds = kN + kN / 16;
+constexpr int kT = 1024;      // threads: 16 waves
+constexpr int kGpw = 4;       // groups of 16 butterflies per wave and stage
+using E = Engine<kN, 16>;     // stage plan (16, 16, 16, 4) and twiddles
+// element pos of the row -> LDS slot: strides 1, 16 and 256 between the 16
+// lanes of a column group map to distinct banks.  Adding a multiple m Ns of
+// Ns in {1, 16, 256, 1024} to a position whose low bits below Ns do not carry
+// moves the slot by m (Ns + Ns/16 + Ns/256): slot(Ns) below.
+__host__ __device__ constexpr int pad(int pos) { return pos + (pos >> 4) + (pos >> 8); }
+__host__ __device__ constexpr int slot(int ns) { return ns + ns / 16 + ns / 256; }
+constexpr int kLds = pad(kN - 1) + 1;
+static_assert(pad(1024 * 15 + 1023) == pad(1023) + 15 * slot(1024), "slot stride of the input reads");
+static_assert(E::plan.nst == 4 && E::plan.radix[0] == 16 && E::plan.radix[1] == 16 && E::plan.radix[2] == 16 &&
+                  E::plan.radix[3] == 4,
+              "stage plan (16, 16, 16, 4) expected");
 } // namespace
 
 struct SplitOp {
   bf16x8 hi, lo;
 };
 
+// Two floats to packed bf16 (round to nearest even; a in the low half).
+__device__ __forceinline__ unsigned pack_bf16(float a, float b)
+{
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{ a, b }, bf16x2));
+}
+
+// hi = bf16(v), lo = bf16(v - hi); the bf16 -> f32 widening of hi is a shift.
 __device__ __forceinline__ SplitOp split8(const float (&v)[8])
 {
-  SplitOp s;
+  u32x4 hi, lo;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    __bf16 h = (__bf16)v[i];
-    s.hi[i] = h;
-    s.lo[i] = (__bf16)(v[i] - (float)h);
+  for (int i = 0; i < 4; ++i) {
+    const float a = v[2 * i], b = v[2 * i + 1];
+    const unsigned p = pack_bf16(a, b);
+    hi[i] = p;
+    lo[i] = pack_bf16(a - __builtin_bit_cast(float, p << 16), b - __builtin_bit_cast(float, p & 0xffff0000u));
   }
+  SplitOp s;
+  s.hi = __builtin_bit_cast(bf16x8, hi);
+  s.lo = __builtin_bit_cast(bf16x8, lo);
   return s;
 }
 
@@ -69,83 +105,101 @@ __constant__ float c_cos16[16] = { 1.0f, 0.92387953251128674f, 0.707106781186547
                                    -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f,
                                    0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f };
 
-// A operands of the real 32x32 DFT-16 matrix for this lane, top half (Yr
-// rows) and bottom half (Yi rows), split into bf16 hi/lo.
-// F[q][r] = cos(2 pi m/16) + i DIR sin(2 pi m/16), m = q r mod 16, and
-// sin(2 pi m/16) = cos(2 pi (m - 4)/16).
-template <int DIR>
-__device__ __forceinline__ void dft16_matrix(int lane_, SplitOp &top, SplitOp &bot)
+// A operands (top: Re rows, bottom: Im rows) from this lane's four
+// coefficients c_{q, 4h + m}, m < 4.
+__device__ __forceinline__ void a_operands(const float2 (&cf)[4], SplitOp &top, SplitOp &bot)
 {
-  const int lane = fresh(lane_);
-  const int q = lane & 15;        // output point (row)
-  const int k0 = 8 * (lane >> 4); // first K index of this lane
   float t[8], b[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int kk = k0 + i;
-    const int m = (q * (kk & 15)) & 15;
-    const float cs = c_cos16[m];
-    const float sn = DIR * c_cos16[(m + 12) & 15];
-    if (kk < 16) { // coefficient on Xr[r]
-      t[i] = cs;
-      b[i] = sn;
-    } else { // coefficient on Xi[r]
-      t[i] = -sn;
-      b[i] = cs;
-    }
+  for (int m = 0; m < 4; ++m) {
+    t[2 * m] = cf[m].x;
+    t[2 * m + 1] = -cf[m].y;
+    b[2 * m] = cf[m].y;
+    b[2 * m + 1] = cf[m].x;
   }
   top = split8(t);
   bot = split8(b);
 }
 
-// One radix-16 stage (stage index S of E's plan) on the planar LDS row.
+// Coefficients of stage S for this lane: row q = lane % 16, inputs
+// r = 4 (lane / 16) + m.  u256[m] = exp(-2 pi i m / 256) (FP64 rounded once).
 template <int S, int DIR>
-__device__ __forceinline__ void mfma_stage(float *re, float *im, const float2 *__restrict__ tw, int wave, int lane_,
-                                           const SplitOp &atop, const SplitOp &abot)
+__device__ __forceinline__ void stage_coefs(int lane, int wave, const float2 *__restrict__ u256, float2 (&cf)[4])
 {
-  const int lane = fresh(lane_);
+  const int q = lane & 15;
+  const int h = lane >> 4;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int r = 4 * h + m;
+    if constexpr (S == 1) {
+      const float2 u = u256[(r * (16 * q + wave)) & 255];
+      cf[m] = make_float2(u.x, DIR < 0 ? u.y : -u.y);
+    } else {
+      const int e = (q * r) & 15;
+      const float s = c_cos16[(e + 12) & 15]; // sin(2 pi e / 16)
+      cf[m] = make_float2(c_cos16[e], DIR < 0 ? -s : s);
+    }
+  }
+}
+
+// Butterfly handled by column c of group gi of this wave in stage S.
+template <int S>
+__device__ __forceinline__ int butterfly(int wave, int gi, int c)
+{
+  if constexpr (S == 1) return wave + 16 * (16 * gi + c); // all with k = wave
+  else return 16 * (kGpw * wave + gi) + c;
+}
+
+// One radix-16 stage on the LDS row.
+template <int S, int DIR>
+__device__ __forceinline__ void mfma_stage(float2 *lds, const float2 *__restrict__ tw, const float2 *__restrict__ u256,
+                                           int wave, int lane_)
+{
   constexpr int Ns = E::plan.ns[S];
   constexpr int twoff = E::plan.twoff[S];
-  static_assert(E::plan.radix[S] == 16, "MFMA stage must be radix 16");
+  const int lane = fresh(lane_);
   const int c = lane & 15;
   const int h = lane >> 4;
-  const int p0 = 8 * (h & 1);
-  const bool part_im = h >= 2;
-  SplitOp bops[kGpw];
+  SplitOp atop, abot;
+  {
+    float2 cf[4];
+    stage_coefs<S, DIR>(lane, wave, u256, cf);
+    a_operands(cf, atop, abot);
+  }
+  f32x4 yr[kGpw], yi[kGpw];
 #pragma unroll
   for (int gi = 0; gi < kGpw; ++gi) {
-    const int j = (wave * kGpw + gi) * 16 + c;
-    const int k = j % Ns;
+    const int j = butterfly<S>(wave, gi, c);
+    const int pj = pad(j) + 4 * h * slot(kN / 16);
     float v[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int r = p0 + i;
-      const int pos = j + r * (kN / 16);
-      float2 x = make_float2(re[pad(pos)], im[pad(pos)]);
-      if (Ns > 1 && r > 0) x = twmul<DIR>(x, tw[twoff + (r - 1) * Ns + k]);
-      v[i] = part_im ? x.y : x.x;
+    for (int m = 0; m < 4; ++m) {
+      const int r = 4 * h + m;
+      float2 x = lds[pj + m * slot(kN / 16)];
+      if constexpr (S == 2) {
+        if (r > 0) x = twmul<DIR>(x, tw[twoff + (r - 1) * Ns + (j % Ns)]);
+      }
+      v[2 * m] = x.x;
+      v[2 * m + 1] = x.y;
     }
-    bops[gi] = split8(v);
+    const SplitOp b = split8(v);
+    f32x4 zr = { 0.f, 0.f, 0.f, 0.f }, zi = { 0.f, 0.f, 0.f, 0.f };
+    zr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(atop.lo, b.hi, zr, 0, 0, 0);
+    zi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abot.lo, b.hi, zi, 0, 0, 0);
+    zr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(atop.hi, b.lo, zr, 0, 0, 0);
+    zi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abot.hi, b.lo, zi, 0, 0, 0);
+    zr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(atop.hi, b.hi, zr, 0, 0, 0);
+    zi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abot.hi, b.hi, zi, 0, 0, 0);
+    yr[gi] = zr;
+    yi[gi] = zi;
   }
   __syncthreads(); // every input read before any output is written
 #pragma unroll
   for (int gi = 0; gi < kGpw; ++gi) {
-    f32x4 yr = { 0.f, 0.f, 0.f, 0.f }, yi = { 0.f, 0.f, 0.f, 0.f };
-    yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(atop.lo, bops[gi].hi, yr, 0, 0, 0);
-    yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(atop.hi, bops[gi].lo, yr, 0, 0, 0);
-    yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(atop.hi, bops[gi].hi, yr, 0, 0, 0);
-    yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abot.lo, bops[gi].hi, yi, 0, 0, 0);
-    yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abot.hi, bops[gi].lo, yi, 0, 0, 0);
-    yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abot.hi, bops[gi].hi, yi, 0, 0, 0);
-    const int j = (wave * kGpw + gi) * 16 + c;
-    const int base = (j / Ns) * Ns * 16 + (j % Ns);
+    const int j = butterfly<S>(wave, gi, c);
+    const int pb = pad((j / Ns) * Ns * 16 + (j % Ns)) + 4 * h * slot(Ns);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int q = 4 * h + i;
-      const int pos = base + q * Ns;
-      re[pad(pos)] = yr[i];
-      im[pad(pos)] = yi[i];
-    }
+    for (int i = 0; i < 4; ++i) lds[pb + i * slot(Ns)] = make_float2(yr[gi][i], yi[gi][i]);
   }
   __syncthreads();
 }
@@ -153,70 +207,62 @@ __device__ __forceinline__ void mfma_stage(float *re, float *im, const float2 *_
 __global__ __launch_bounds__(kT) void k_bf16x3_fastconv(const float2 *__restrict__ x, long ldx,
                                                          const float2 *__restrict__ w, float2 *__restrict__ y,
                                                          long ldy, long rows, const float2 *__restrict__ tw,
-                                                         float scale)
+                                                         const float2 *__restrict__ u256, float scale)
 {
-  __shared__ float re[kLds], im[kLds];
+  __shared__ float2 lds[kLds];
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
   const long row = blockIdx.x;
   if (row >= rows) return; // one row per workgroup: the whole group leaves together
-  const float2 *xr = x + row * ldx;
-  float2 *yr = y + row * ldy;
+  const rsrc_t rx = make_rsrc(x + row * ldx, (uint32_t)kN * 8u);
+  const rsrc_t ry = make_rsrc(y + row * ldy, (uint32_t)kN * 8u);
+  const rsrc_t rw = make_rsrc(w, (uint32_t)kN * 8u);
   const rsrc_t rtw = make_rsrc(tw, (uint32_t)(E::plan.twtotal + 1) * 8u);
+  const uint32_t vt = (uint32_t)t * 8u;
 
+  {
+    const int pt = pad(t);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int pos = t + kT * q;
-    float2 v = xr[pos];
-    re[pad(pos)] = v.x;
-    im[pad(pos)] = v.y;
+    for (int q = 0; q < 16; ++q) lds[pt + q * slot(kT)] = bload_nt(rx, vt, (uint32_t)(kT * q) * 8u);
   }
   __syncthreads();
-  SplitOp ftop, fbot;
-  dft16_matrix<-1>(lane, ftop, fbot);
-  mfma_stage<0, -1>(re, im, tw, wave, lane, ftop, fbot);
-  mfma_stage<1, -1>(re, im, tw, wave, lane, ftop, fbot);
-  mfma_stage<2, -1>(re, im, tw, wave, lane, ftop, fbot);
-  // radix-4 stage (FP32, VALU) in slot form, spectrum multiply
-  float2 v[16];
-  int ts = fresh(t);
+  mfma_stage<0, -1>(lds, tw, u256, wave, lane);
+  mfma_stage<1, -1>(lds, tw, u256, wave, lane);
+  mfma_stage<2, -1>(lds, tw, u256, wave, lane);
+  // radix-4 stage (FP32, VALU) in slot form, spectrum multiply, back in place
+  {
+    float2 v[16];
+    const int ts = fresh(t);
+    const int pt = pad(ts);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int pos = ts + kT * q;
-    v[q] = make_float2(re[pad(pos)], im[pad(pos)]);
-  }
-  E::template stage<3, -1>(v, (float2 *)nullptr, ts, rtw);
+    for (int q = 0; q < 16; ++q) v[q] = lds[pt + q * slot(kT)];
+    E::template stage<3, -1>(v, (float2 *)nullptr, ts, rtw);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = cscale(cmul(v[q], w[ts + kT * q]), scale);
-  __syncthreads(); // all slot reads done before the planar row is rewritten
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int pos = ts + kT * q;
-    re[pad(pos)] = v[q].x;
-    im[pad(pos)] = v[q].y;
+    for (int q = 0; q < 16; ++q)
+      lds[pt + q * slot(kT)] = cscale(cmul(v[q], bload(rw, vt, (uint32_t)(kT * q) * 8u)), scale);
   }
   __syncthreads();
-  SplitOp itop, ibot;
-  dft16_matrix<+1>(lane, itop, ibot);
-  mfma_stage<0, +1>(re, im, tw, wave, lane, itop, ibot);
-  mfma_stage<1, +1>(re, im, tw, wave, lane, itop, ibot);
-  mfma_stage<2, +1>(re, im, tw, wave, lane, itop, ibot);
-  ts = fresh(t);
+  mfma_stage<0, +1>(lds, tw, u256, wave, lane);
+  mfma_stage<1, +1>(lds, tw, u256, wave, lane);
+  mfma_stage<2, +1>(lds, tw, u256, wave, lane);
+  {
+    float2 v[16];
+    const int ts = fresh(t);
+    const int pt = pad(ts);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int pos = ts + kT * q;
-    v[q] = make_float2(re[pad(pos)], im[pad(pos)]);
+    for (int q = 0; q < 16; ++q) v[q] = lds[pt + q * slot(kT)];
+    E::template stage<3, +1>(v, (float2 *)nullptr, ts, rtw);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) bstore_nt(ry, v[q], vt, (uint32_t)(kT * q) * 8u);
   }
-  E::template stage<3, +1>(v, (float2 *)nullptr, ts, rtw);
-#pragma unroll
-  for (int q = 0; q < 16; ++q) yr[ts + kT * q] = v[q];
 }
 
 hipError_t launch_bf16x3_fastconv(const float2 *x, long ldx, const float2 *w, float2 *y, long ldy, long rows,
-                                  const float2 *tw, float scale, hipStream_t s)
+                                  const float2 *tw, const float2 *u256, float scale, hipStream_t s)
 {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bf16x3_fastconv, dim3((unsigned)rows), dim3(kT), 0, s, x, ldx, w, y, ldy, rows, tw, scale);
+  hipLaunchKernelGGL(k_bf16x3_fastconv, dim3((unsigned)rows), dim3(kT), 0, s, x, ldx, w, y, ldy, rows, tw, u256,
+                     scale);
   return hipGetLastError();
 }
 

@@ -131,6 +131,12 @@ int LineFFT::init(long n_, int variant_)
     if ((st = tlo.alloc(lo * 8)) || (st = thi.alloc(hi * 8))) return st;
     hipError_t e = launch_unit_roots(n, lo, (float2 *)tlo.p, nullptr);
     if (e == hipSuccess) e = launch_unit_roots(n / lo, hi, (float2 *)thi.p, nullptr);
+    if (e == hipSuccess && n <= kFsFullTwiddleMax) {
+      // one coalesced load per element instead of two gathered ones and a
+      // product (n entries, read from L2 by every row)
+      if ((st = tfull.alloc(n * 8))) return st;
+      e = launch_fs_full_twiddles(n1, n2, (float2 *)tfull.p, nullptr);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return hip_fail(e, "four-step twiddles");
     kind = FOURSTEP;
@@ -150,6 +156,7 @@ static FsArgs fs_args(const LineFFT &f)
   a.tlo = (const float2 *)f.tlo.p;
   a.thi = (const float2 *)f.thi.p;
   a.sbits = f.sbits;
+  a.tfull = (const float2 *)f.tfull.p;
   a.scale = 1.0f;
   return a;
 }
@@ -166,7 +173,7 @@ const char *LineFFT::name() const
 
 size_t LineFFT::device_bytes() const
 {
-  return tw.bytes + tw1.bytes + tw2.bytes + tlo.bytes + thi.bytes + work.bytes + wp.bytes + tw_outer.bytes +
+  return tw.bytes + tw1.bytes + tw2.bytes + tlo.bytes + thi.bytes + tfull.bytes + work.bytes + wp.bytes + tw_outer.bytes +
          tw_wave.bytes;
 }
 
@@ -573,6 +580,10 @@ extern "C" int ovx_fastconv_plan_create(ovx_plan *plan, long rows, long n, float
     if (st) return st;
     if (p->line.kind != LineFFT::LDS || p->line.rc.p != 16 || p->line.rc.plan.twtotal != bf16x3_twiddle_count())
       return fail(OVX_ERR_STATE, "BF16x3 variant: unexpected stage plan");
+    if ((st = p->u256.alloc(256 * 8))) return st;
+    hipError_t e = launch_unit_roots(256, 256, (float2 *)p->u256.p, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hip_fail(e, "BF16x3 tables");
     *plan = p.release();
     return OVX_OK;
   }
@@ -596,8 +607,9 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
   hipStream_t s = (hipStream_t)stream;
   if (p->bf16x3) {
     if (ldw != 0) return fail(OVX_ERR_UNSUPPORTED, "BF16x3 variant takes one spectrum vector");
+    if (!p->line.tables_ready() || !p->u256.p) return fail(OVX_ERR_STATE, "BF16x3 tables missing");
     hipError_t e = launch_bf16x3_fastconv((const float2 *)x, ldx, (const float2 *)w, (float2 *)y, ldy, rows,
-                                          (const float2 *)p->line.tw.p, p->scale, s);
+                                          (const float2 *)p->line.tw.p, (const float2 *)p->u256.p, p->scale, s);
     if (e != hipSuccess) return hip_fail(e, "bf16x3 fastconv launch");
     return OVX_OK;
   }

@@ -16,9 +16,13 @@
 // to global memory.  Every other stage exchanges through LDS: one write of P
 // values, a barrier, one read of P values in slot form.
 //
-// LDS rows are padded by one element every 16 (phys = pos + pos/16), which
-// keeps every 16-lane group of the exchange patterns used here on distinct
-// banks.
+// LDS layout of a line (LdsLayout): gfx950's LDS has 64 banks of 4 bytes and
+// serves a 64-bit access for 32 lanes per cycle, so the 32 lanes of each
+// half-wave must hit 32 distinct 8-byte slots.  Lines handled by at least 32
+// threads use the XOR swizzle slot = pos ^ ((pos >> 4) & 31), a permutation
+// inside every aligned block of 32 elements that keeps the Stockham write
+// strides (1, R, Ns) and the slot-form reads on distinct slots; shorter lines
+// (several per workgroup) use one pad element every 16.
 #pragma once
 #include "ovx_device.hpp"
 
@@ -65,7 +69,17 @@ constexpr RadixPlan make_plan(int N, int P)
   return p;
 }
 
-constexpr int lds_pad(int pos) { return pos + (pos >> 4); }
+constexpr int lds_row_elems(int N) { return N >= 16 ? N + N / 16 : N; }
+
+template <int N, int P>
+struct LdsLayout {
+  static constexpr bool kSwizzle = (N / P) >= 32 && N % 32 == 0;
+  static constexpr int elems = kSwizzle ? N : lds_row_elems(N);
+  __host__ __device__ static constexpr int idx(int pos)
+  {
+    return kSwizzle ? (pos ^ ((pos >> 4) & 31)) : pos + (pos >> 4);
+  }
+};
 
 // Pointer into LDS with the address space spelled out (for code that is not
 // inlined into the kernel owning the __shared__ array), and element access
@@ -82,8 +96,6 @@ __device__ __forceinline__ float2 lds_ld(lds_ptr p, int i)
   f2v t = p[i];
   return make_float2(t.x, t.y);
 }
-constexpr int lds_row_elems(int N) { return N >= 16 ? N + N / 16 : N; }
-
 // Synchronisation between the LDS write and read halves of an exchange.
 // Block scope: every wave of the workgroup takes part in the line.
 // Wave scope: the line lives in one wavefront, whose LDS operations complete
@@ -105,7 +117,8 @@ struct Engine {
   static constexpr int T = N / P;
   static constexpr RadixPlan plan = make_plan(N, P);
   static constexpr int NST = plan.nst;
-  static constexpr int LROW = lds_row_elems(N);
+  using Layout = LdsLayout<N, P>;
+  static constexpr int LROW = Layout::elems;
   static_assert(N % P == 0, "P must divide N");
   static_assert(plan.nst > 0 || N == 1, "no radix plan for N with this P");
 
@@ -139,13 +152,13 @@ struct Engine {
       } else {
         const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds_st(lds, lds_pad(base + r * Ns), a[r]);
+        for (int r = 0; r < R; ++r) lds_st(lds, Layout::idx(base + r * Ns), a[r]);
       }
     }
     if constexpr (!last) {
       exchange_sync<WAVE>();
 #pragma unroll
-      for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, lds_pad(t + T * q));
+      for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, Layout::idx(t + T * q));
       exchange_sync<WAVE>();
     }
   }

@@ -100,6 +100,25 @@ hipError_t launch_fs_row(int mode, const FsArgs &a, long rows, hipStream_t s)
   }
 }
 
+// tfull[k1 * n2 + c] = W_N^{k1 c}, N = n1 n2, from FP64 values rounded once.
+__global__ void k_fs_full_twiddles(long n1, long n2, float2 *__restrict__ out)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = n1 * n2;
+  if (i >= n) return;
+  const long m = (i / n2) * (i % n2); // < n1 n2
+  double sn, cs;
+  sincospi(-2.0 * (double)m / (double)n, &sn, &cs);
+  out[i] = make_float2((float)cs, (float)sn);
+}
+
+hipError_t launch_fs_full_twiddles(long n1, long n2, float2 *out, hipStream_t s)
+{
+  long n = n1 * n2;
+  hipLaunchKernelGGL(k_fs_full_twiddles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n1, n2, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_fs_permute(const float2 *w, long ldw, float2 *wp, long n1, long n2, long rows, float scale,
                              hipStream_t s)
 {

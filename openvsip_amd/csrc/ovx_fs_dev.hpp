@@ -11,7 +11,7 @@ namespace ovx {
 // line index land on different banks.
 template <int L>
 struct FsLds {
-  static constexpr int LROW = lds_row_elems(L) | 1;
+  static constexpr int LROW = LdsLayout<L, 16>::elems | 1;
 };
 
 __device__ __forceinline__ float2 big_twiddle(const float2 *__restrict__ tlo, const float2 *__restrict__ thi,
@@ -50,12 +50,21 @@ __device__ __forceinline__ void fs_col_body(const FsArgs &a, long row, long grou
   if constexpr (MODE == 0) E::template fft<-1>(v, lds, t, rtw);
   else E::template fft<+1>(v, lds, t, rtw);
   if constexpr (MODE == 0 || MODE == 2) {
+    if (a.tfull) { // lanes of consecutive columns read consecutive entries
+      const float2 *tf = a.tfull + n2;
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-      const uint32_t k1 = (uint32_t)(t + T * q);
-      const uint32_t m = (uint32_t)n2 * k1; // < N1*N2: no reduction needed
-      float2 w = big_twiddle(a.tlo, a.thi, a.sbits, m);
-      v[q] = (MODE == 0) ? cmul(v[q], w) : cmulc(v[q], w);
+      for (int q = 0; q < P; ++q) {
+        float2 w = tf[(long)(t + T * q) * N2];
+        v[q] = (MODE == 0) ? cmul(v[q], w) : cmulc(v[q], w);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const uint32_t k1 = (uint32_t)(t + T * q);
+        const uint32_t m = (uint32_t)n2 * k1; // < N1*N2: no reduction needed
+        float2 w = big_twiddle(a.tlo, a.thi, a.sbits, m);
+        v[q] = (MODE == 0) ? cmul(v[q], w) : cmulc(v[q], w);
+      }
     }
   } else {
 #pragma unroll
@@ -94,11 +103,17 @@ __device__ __forceinline__ void fs_row_body(const FsArgs &a, long row, long grou
 #pragma unroll
     for (int q = 0; q < P; ++q) v[q] = cmul(v[q], wp[t + T * q]);
     E::template fft<+1>(v, lds, t, rtw);
+    if (a.tfull) {
+      const float2 *tf = a.tfull + k1 * L;
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-      const uint32_t n2 = (uint32_t)(t + T * q);
-      const uint32_t m = n2 * (uint32_t)k1;
-      v[q] = cmulc(v[q], big_twiddle(a.tlo, a.thi, a.sbits, m));
+      for (int q = 0; q < P; ++q) v[q] = cmulc(v[q], tf[t + T * q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const uint32_t n2 = (uint32_t)(t + T * q);
+        const uint32_t m = n2 * (uint32_t)k1;
+        v[q] = cmulc(v[q], big_twiddle(a.tlo, a.thi, a.sbits, m));
+      }
     }
     float2 *dst = a.out + row * a.ld_out + k1 * L;
 #pragma unroll

@@ -12,11 +12,13 @@ struct Hier {
   static constexpr int R0 = N / NW;      // outer radix = waves per row
   static constexpr int TR = 64 * R0;     // threads per row
   static constexpr int M0 = 16 / R0;     // outer butterflies per thread
-  static constexpr int LSL = NW + NW / 16; // LDS slice stride (padded)
-  static constexpr int LROW = R0 * LSL;  // = N + N/16
+  using WE = Engine<NW, 16, true>;
+  static constexpr int LSL = WE::LROW;   // LDS slice stride
+  static constexpr int LROW = R0 * LSL;
   static constexpr int B = TR >= 256 ? 1 : 256 / TR; // rows per block
   static constexpr int NT = TR * B;
-  using WE = Engine<NW, 16, true>;
+  // slot of row position pos (slice pos / NW, position pos % NW in it)
+  __device__ static constexpr int lidx(int pos) { return (pos / NW) * LSL + WE::Layout::idx(pos % NW); }
   static_assert(R0 * NW == N && R0 >= 1 && R0 <= 16, "N must be 1024 * {1,2,4,8,16}");
 };
 
@@ -40,7 +42,7 @@ __device__ __forceinline__ void outer_forward(const rsrc_t &rx, uint32_t vx, boo
       for (int k1 = 1; k1 < R0; ++k1) a[k1] = cmul(a[k1], two[(k1 - 1) * NW + j]);
     }
 #pragma unroll
-    for (int k1 = 0; k1 < R0; ++k1) lds_st(lds, k1 * H::LSL + lds_pad(j), a[k1]);
+    for (int k1 = 0; k1 < R0; ++k1) lds_st(lds, k1 * H::LSL + H::WE::Layout::idx(j), a[k1]);
   }
 }
 
@@ -56,7 +58,7 @@ __device__ __forceinline__ void outer_inverse_store(const rsrc_t &ry, uint32_t v
     const int j = t + H::TR * m;
     float2 a[R0];
 #pragma unroll
-    for (int k1 = 0; k1 < R0; ++k1) a[k1] = lds_ld(lds, k1 * H::LSL + lds_pad(j));
+    for (int k1 = 0; k1 < R0; ++k1) a[k1] = lds_ld(lds, k1 * H::LSL + H::WE::Layout::idx(j));
     if constexpr (R0 > 1) Dft<R0, +1>::run(a);
     if (scale != 1.0f) {
 #pragma unroll
@@ -102,7 +104,7 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
   float2 u[16];
   if constexpr (R0 > 1) {
 #pragma unroll
-    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, lds_pad(l + 64 * s));
+    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, H::WE::Layout::idx(l + 64 * s));
   } else {
 #pragma unroll
     for (int s = 0; s < 16; ++s)
@@ -135,7 +137,7 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
       for (int s = 0; s < 16; ++s) u[s] = cmulc(u[s], a.tw_outer[(k1 - 1) * NW + l + 64 * s]);
     }
 #pragma unroll
-    for (int s = 0; s < 16; ++s) lds_st(sl, lds_pad(l + 64 * s), u[s]);
+    for (int s = 0; s < 16; ++s) lds_st(sl, H::WE::Layout::idx(l + 64 * s), u[s]);
     __syncthreads();
     outer_inverse_store<N, LP>(ry, vy, valid, lds, t, 1.0f);
   } else {
@@ -172,7 +174,7 @@ __device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int 
   float2 u[16];
   if constexpr (R0 > 1) {
 #pragma unroll
-    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, lds_pad(l + 64 * s));
+    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, H::WE::Layout::idx(l + 64 * s));
   } else {
 #pragma unroll
     for (int s = 0; s < 16; ++s)
@@ -186,11 +188,11 @@ __device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int 
   if constexpr (R0 > 1) {
     __syncthreads(); // every wave is done with its slice
 #pragma unroll
-    for (int s = 0; s < 16; ++s) lds_st(lds, lds_pad(k1 + R0 * (l + 64 * s)), u[s]);
+    for (int s = 0; s < 16; ++s) lds_st(lds, H::lidx(k1 + R0 * (l + 64 * s)), u[s]);
     __syncthreads();
     if (valid) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) bstore_nt(ry, lds_ld(lds, lds_pad(t + H::TR * q)), vy, (uint32_t)(H::TR * q) * 8u);
+      for (int q = 0; q < 16; ++q) bstore_nt(ry, lds_ld(lds, H::lidx(t + H::TR * q)), vy, (uint32_t)(H::TR * q) * 8u);
     }
   } else {
     if (valid) {
@@ -221,10 +223,10 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
   if constexpr (R0 > 1) {
 #pragma unroll
     for (int q = 0; q < 16; ++q)
-      lds_st(lds, lds_pad(t + H::TR * q), valid ? bload_nt(rx, vx, (uint32_t)(H::TR * q) * 8u) : make_float2(0.f, 0.f));
+      lds_st(lds, H::lidx(t + H::TR * q), valid ? bload_nt(rx, vx, (uint32_t)(H::TR * q) * 8u) : make_float2(0.f, 0.f));
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 16; ++s) u[s] = lds_ld(lds, lds_pad(k1 + R0 * (l + 64 * s)));
+    for (int s = 0; s < 16; ++s) u[s] = lds_ld(lds, H::lidx(k1 + R0 * (l + 64 * s)));
     __syncthreads(); // all natural-order reads done before slices are reused
   } else {
 #pragma unroll
@@ -238,7 +240,7 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
       for (int s = 0; s < 16; ++s) u[s] = cmulc(u[s], a.tw_outer[(k1 - 1) * NW + l + 64 * s]);
     }
 #pragma unroll
-    for (int s = 0; s < 16; ++s) lds_st(sl, lds_pad(l + 64 * s), u[s]);
+    for (int s = 0; s < 16; ++s) lds_st(sl, H::WE::Layout::idx(l + 64 * s), u[s]);
     __syncthreads();
     outer_inverse_store<N, LP>(ry, vy, valid, lds, t, a.scale);
   } else {

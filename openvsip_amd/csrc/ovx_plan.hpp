@@ -39,7 +39,7 @@ struct LineFFT {
   long n1 = 0, n2 = 0;
   int sbits = 0;
   RowConfig rc1, rc2;
-  DevBuf tw1, tw2, tlo, thi;
+  DevBuf tw1, tw2, tlo, thi, tfull;
   mutable DevBuf work, wp; // Fftm workspace, permuted spectrum
 
   int init(long n, int variant);
@@ -82,6 +82,7 @@ struct ovx_plan_s {
   ovx::DevBuf stage_in, stage_out, stage_w;
   ovx::CorrState *corr = nullptr;
   bool bf16x3 = false; // fastconv radix-16 stages on MFMA (precision variant)
+  ovx::DevBuf u256;    // its 256th roots of unity
 
   ~ovx_plan_s();
   const char *name() const;

@@ -331,7 +331,7 @@ int Server::launch_plain(Slot &s, Req &q)
 
 void Server::scheduler()
 {
-  hipSetDevice(device);
+  (void)hipSetDevice(device);
   for (;;) {
     std::vector<Req *> batch;
     Slot *s = nullptr;
@@ -411,7 +411,7 @@ void Server::scheduler()
 
 void Server::completer()
 {
-  hipSetDevice(device);
+  (void)hipSetDevice(device);
   for (;;) {
     Slot *s = nullptr;
     {
