@@ -128,6 +128,25 @@ int ovx_fftm_out_of_place(ovx_plan plan, const void *in, long in_s0, long in_s1,
 int ovx_fftm_in_place(ovx_plan plan, void *inout, long s0, long s1, long rows,
                       long cols, void *stream);
 
+/* ---- real Fftm (real <-> complex, multiple 1-D transforms) ------------- */
+/* Plan for a rows x cols *real* domain, transforms of even length n = cols
+   (OVX_ROW) or rows (OVX_COL); the complex side holds n/2 + 1 values per line.
+   OVX_FFT_FWD: real -> complex, replaces fftm_backend<T, complex<T>, A,
+   fft_fwd> (backend.hpp:598-640; FFTW r2c in fftw.cpp:778-843).
+   OVX_FFT_INV: complex -> real, replaces fftm_backend<complex<T>, T, A,
+   fft_inv> (backend.hpp:642-690; FFTW c2r in fftw.cpp:845-903): unnormalised
+   like FFTW's c2r, the imaginary parts of frequencies 0 and n/2 ignored.
+   The result is multiplied by `scale` in the kernel. */
+int ovx_fftm_real_plan_create(ovx_plan *plan, long rows, long cols, int axis,
+                              int direction, float scale, unsigned flags);
+/* out_of_place of those backends: strides in elements of each side's type
+   (float on the real side, complex<float> on the complex side), unit stride
+   along the transform axis, even real line stride, 8-byte aligned real data;
+   rows x cols is the real domain. */
+int ovx_fftm_real_out_of_place(ovx_plan plan, const void *in, long in_s0,
+                               long in_s1, void *out, long out_s0, long out_s1,
+                               long rows, long cols, void *stream);
+
 /* ---- Fft (one 1-D transform) ------------------------------------------- */
 /* vsip::Fft<const_Vector,C,C,dir> construction (fft.hpp:75-103). */
 int ovx_fft_plan_create(ovx_plan *plan, long n, int direction, float scale,

@@ -53,6 +53,8 @@ SIGNATURES = {
     "ovx_fftm_plan_create": (_I, [_PP, _L, _L, _I, _I, _F, _U]),
     "ovx_fftm_out_of_place": (_I, [_P, _P, _L, _L, _P, _L, _L, _L, _L, _P]),
     "ovx_fftm_in_place": (_I, [_P, _P, _L, _L, _L, _L, _P]),
+    "ovx_fftm_real_plan_create": (_I, [_PP, _L, _L, _I, _I, _F, _U]),
+    "ovx_fftm_real_out_of_place": (_I, [_P, _P, _L, _L, _P, _L, _L, _L, _L, _P]),
     "ovx_fft_plan_create": (_I, [_PP, _L, _I, _F, _U]),
     "ovx_fft_out_of_place": (_I, [_P, _P, _L, _P, _L, _L, _P]),
     "ovx_fft_in_place": (_I, [_P, _P, _L, _L, _P]),

@@ -118,6 +118,43 @@ class Fftm(_Plan):
         return x
 
 
+class FftmReal(_Plan):
+    """Real <-> complex Fftm over a rows x cols real domain.  FFT_FWD maps
+    real float32 lines of even length n to n/2 + 1 complex values; FFT_INV
+    maps n/2 + 1 complex values back to n real ones (unnormalised, times
+    `scale`)."""
+
+    def __init__(self, rows, cols, axis=ROW, direction=FFT_FWD, scale=1.0, variant=None):
+        super().__init__()
+        self.rows, self.cols, self.axis, self.direction, self.scale = rows, cols, axis, direction, scale
+        check(lib().ovx_fftm_real_plan_create(ctypes.byref(self._h), rows, cols, axis, direction, scale,
+                                              _flags(variant)))
+
+    def complex_shape(self):
+        if self.axis == ROW:
+            return (self.rows, self.cols // 2 + 1)
+        return (self.rows // 2 + 1, self.cols)
+
+    def __call__(self, x, out=None):
+        torch = _torch()
+        if out is None:
+            if self.direction == FFT_FWD:
+                shape = self.complex_shape()
+                dtype = torch.complex64
+            else:
+                shape = (self.rows, self.cols)
+                dtype = torch.float32
+            if self.axis == ROW:
+                out = torch.empty(shape, dtype=dtype, device=x.device)
+            else:  # column lines are contiguous: column-major storage
+                out = torch.empty(shape[::-1], dtype=dtype, device=x.device).t()
+        if not (x.is_cuda and out.is_cuda):
+            raise ValueError("real Fftm tensors must live on the device")
+        check(lib().ovx_fftm_real_out_of_place(self._h, _ptr(x), x.stride(0), x.stride(1), _ptr(out),
+                                               out.stride(0), out.stride(1), self.rows, self.cols, _stream()))
+        return out
+
+
 class Fft(_Plan):
     def __init__(self, n, direction=FFT_FWD, scale=1.0):
         super().__init__()

@@ -17,6 +17,7 @@
 
 #include "../../include/ovxhip.h"
 #include "ovx_bf16.hpp"
+#include "ovx_real.hpp"
 #include "ovx_fourstep.hpp"
 #include "ovx_hier.hpp"
 #include "ovx_internal.hpp"
@@ -555,6 +556,96 @@ extern "C" int ovx_fft_out_of_place(ovx_plan p, const void *in, long in_stride, 
 extern "C" int ovx_fft_in_place(ovx_plan p, void *inout, long stride, long n, void *stream)
 {
   return ovx_fft_out_of_place(p, inout, stride, inout, stride, n, stream);
+}
+
+// ---------------------------------------------------------------- real Fftm
+extern "C" int ovx_fftm_real_plan_create(ovx_plan *plan, long rows, long cols, int axis, int direction, float scale,
+                                         unsigned flags)
+{
+  if (!plan) return fail(OVX_ERR_INVALID, "null plan pointer");
+  *plan = nullptr;
+  if (rows < 0 || cols < 0) return fail(OVX_ERR_INVALID, "negative size");
+  if (axis != OVX_ROW && axis != OVX_COL) return fail(OVX_ERR_INVALID, "axis must be OVX_ROW or OVX_COL");
+  if (!is_dir(direction)) return fail(OVX_ERR_INVALID, "direction must be OVX_FFT_FWD or OVX_FFT_INV");
+  const long n = axis == OVX_ROW ? cols : rows;
+  if (n % 2) return fail(OVX_ERR_UNSUPPORTED, "real transform length %ld is odd", n);
+  std::unique_ptr<ovx_plan_s> p(new (std::nothrow) ovx_plan_s());
+  if (!p) return fail(OVX_ERR_NOMEM, "host allocation");
+  p->kind = PLAN_FFTM_REAL;
+  p->rows = rows;
+  p->cols = cols;
+  p->axis = axis;
+  p->dir = direction;
+  p->scale = scale;
+  p->flags = flags;
+  if (n > 0) {
+    const long h = n / 2;
+    int st = p->line.init(h, variant_of(flags, h, axis == OVX_ROW ? rows : cols));
+    if (st) return st;
+    if ((st = p->wtab.alloc((size_t)(h / 2 + 1) * 8))) return st;
+    hipError_t e = launch_unit_roots(n, h / 2 + 1, (float2 *)p->wtab.p, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hip_fail(e, "real Fftm tables");
+  }
+  *plan = p.release();
+  return OVX_OK;
+}
+
+// line length / count / stride of one side of a real Fftm: `len` values
+// per line along the plan's axis, unit stride along it.
+static int real_side(ovx_plan p, long s0, long s1, long len, long *lines, long *ld)
+{
+  if (p->axis == OVX_ROW) {
+    if (s1 != 1 && len > 1) return fail(OVX_ERR_UNSUPPORTED, "column stride must be 1 for a row Fftm");
+    *lines = p->rows;
+    *ld = s0;
+  } else {
+    if (s0 != 1 && len > 1) return fail(OVX_ERR_UNSUPPORTED, "row stride must be 1 for a column Fftm");
+    *lines = p->cols;
+    *ld = s1;
+  }
+  if (*lines > 1 && *ld < len) return fail(OVX_ERR_INVALID, "line stride %ld shorter than the line %ld", *ld, len);
+  return OVX_OK;
+}
+
+extern "C" int ovx_fftm_real_out_of_place(ovx_plan p, const void *in, long in_s0, long in_s1, void *out, long out_s0,
+                                          long out_s1, long rows, long cols, void *stream)
+{
+  int st = check_plan(p, PLAN_FFTM_REAL);
+  if (st) return st;
+  if (!in || !out) return fail(OVX_ERR_INVALID, "null data pointer");
+  if (rows != p->rows || cols != p->cols)
+    return fail(OVX_ERR_INVALID, "size %ldx%ld does not match the plan's %ldx%ld", rows, cols, p->rows, p->cols);
+  const bool fwd = p->dir == OVX_FFT_FWD;
+  const long n = p->axis == OVX_ROW ? cols : rows, h = n / 2;
+  long lines, ldr, ldc;
+  // real side: n floats per line; complex side: h + 1 values per line
+  if ((st = real_side(p, fwd ? in_s0 : out_s0, fwd ? in_s1 : out_s1, n, &lines, &ldr))) return st;
+  if ((st = real_side(p, fwd ? out_s0 : in_s0, fwd ? out_s1 : in_s1, h + 1, &lines, &ldc))) return st;
+  if (n == 0 || lines == 0) return OVX_OK;
+  if (lines > 1 && (ldr % 2))
+    return fail(OVX_ERR_UNSUPPORTED, "real line stride %ld must be even (lines are read as complex pairs)", ldr);
+  const void *real_ptr = fwd ? in : out;
+  if (((uintptr_t)real_ptr) % 8) return fail(OVX_ERR_UNSUPPORTED, "real data must be 8-byte aligned");
+  if (!p->line.tables_ready() || !p->wtab.p) return fail(OVX_ERR_STATE, "real Fftm tables missing");
+  hipStream_t s = (hipStream_t)stream;
+  const long ldz = lines > 1 ? ldr / 2 : h; // real line as h complex values
+  auto run = [&](const void *din, void *dout) -> int {
+    if (fwd) {
+      int r = p->line.transform(-1, (const float2 *)din, ldz, (float2 *)dout, ldc, lines, 1.0f, s);
+      if (r) return r;
+      hipError_t e = launch_r2c_post((float2 *)dout, ldc, lines, h, (const float2 *)p->wtab.p, p->scale, s);
+      return e == hipSuccess ? OVX_OK : hip_fail(e, "r2c pass");
+    }
+    hipError_t e = launch_c2r_pre((const float2 *)din, ldc, (float2 *)dout, ldz, lines, h, (const float2 *)p->wtab.p, s);
+    if (e != hipSuccess) return hip_fail(e, "c2r pass");
+    return p->line.transform(+1, (const float2 *)dout, ldz, (float2 *)dout, ldz, lines, p->scale, s);
+  };
+  if (p->flags & OVX_FLAG_HOST_POINTERS) {
+    const size_t rb = ((size_t)(lines - 1) * ldr + n) * 4, cb = ((size_t)(lines - 1) * ldc + h + 1) * 8;
+    return with_staging(p, in, fwd ? rb : cb, out, fwd ? cb : rb, s, run);
+  }
+  return run(in, out);
 }
 
 // ---------------------------------------------------------------- fastconv

@@ -111,6 +111,8 @@ const char *ovx_plan_s::name() const
            : line.kind == LineFFT::FOURSTEP ? "fastconv:four-step-3pass"
            : line.kind == LineFFT::HIER     ? "fastconv:fused-hierarchical-lds"
                                             : "fastconv:none";
+  case PLAN_FFTM_REAL:
+    return dir == OVX_FFT_FWD ? "fftm-real:r2c-half-length" : "fftm-real:c2r-half-length";
   case PLAN_CORR: return "corr:overlap-save";
   case PLAN_CONV: return "conv:overlap-save";
   default: return "unknown";
@@ -119,7 +121,7 @@ const char *ovx_plan_s::name() const
 
 size_t ovx_plan_s::device_bytes() const
 {
-  size_t b = line.device_bytes() + stage_in.bytes + stage_out.bytes + stage_w.bytes + u256.bytes;
+  size_t b = line.device_bytes() + stage_in.bytes + stage_out.bytes + stage_w.bytes + u256.bytes + wtab.bytes;
   if (corr)
     b += corr->fftL.device_bytes() + corr->hpad.bytes + corr->H.bytes + corr->ref_stage.bytes +
          corr->in_stage.bytes + corr->out_stage.bytes;

@@ -65,7 +65,7 @@ struct LineFFT {
   bool tables_ready() const;
 };
 
-enum PlanKind { PLAN_FFTM = 1, PLAN_FFT, PLAN_FASTCONV, PLAN_CORR, PLAN_CONV };
+enum PlanKind { PLAN_FFTM = 1, PLAN_FFT, PLAN_FASTCONV, PLAN_CORR, PLAN_CONV, PLAN_FFTM_REAL };
 
 struct CorrState; // ovx_corr.cpp
 
@@ -83,6 +83,7 @@ struct ovx_plan_s {
   ovx::CorrState *corr = nullptr;
   bool bf16x3 = false; // fastconv radix-16 stages on MFMA (precision variant)
   ovx::DevBuf u256;    // its 256th roots of unity
+  ovx::DevBuf wtab;    // real Fftm: exp(-2 pi i k / n), k <= n/4
 
   ~ovx_plan_s();
   const char *name() const;

@@ -105,6 +105,35 @@ def fftm64(x, direction=FWD, scale=1.0):
     return out
 
 
+def rfftm64(x, scale=1.0):
+    """Real-input Fftm by rows (FFTW r2c as the reference's real fftm calls it,
+    src/ovxx/fftw/fftw.cpp:778-843): n real samples -> n/2 + 1 complex
+    values, FP64, times scale."""
+    x = np.atleast_2d(np.asarray(x, dtype=np.float64))
+    n = x.shape[1]
+    return fftm64(x.astype(np.complex128))[:, : n // 2 + 1] * scale
+
+
+def hermitian_extend(xh, n):
+    """Full n-point spectrum from the half spectrum X[0 .. n/2] of a real
+    signal (imaginary parts of frequencies 0 and n/2 dropped, as FFTW's c2r
+    and the reference's real inverse Fftm assume them zero)."""
+    xh = np.atleast_2d(np.asarray(xh, dtype=np.complex128)).copy()
+    h = n // 2
+    xh[:, 0] = xh[:, 0].real
+    xh[:, h] = xh[:, h].real
+    full = np.empty((xh.shape[0], n), dtype=np.complex128)
+    full[:, : h + 1] = xh[:, : h + 1]
+    full[:, h + 1:] = np.conj(xh[:, 1:h][:, ::-1])
+    return full
+
+
+def irfftm64(xh, n, scale=1.0):
+    """Complex-to-real Fftm by rows (FFTW c2r, src/ovxx/fftw/fftw.cpp:845-903),
+    unnormalised, FP64, times scale."""
+    return fftm64(hermitian_extend(xh, n), INV).real * scale
+
+
 def fftm_port(x, direction=FWD, scale=1.0):
     x = _c64(np.atleast_2d(x))
     out = np.empty_like(x)

@@ -54,6 +54,11 @@ def test_invalid_arguments_return_status_not_crash():
     assert lib.ovx_fftm_plan_create(ctypes.byref(h), 4, 4, 5, -2, 1.0, 0) == _lib.OVX_ERR_INVALID
     assert lib.ovx_fftm_plan_create(ctypes.byref(h), 4, 4, 0, 3, 1.0, 0) == _lib.OVX_ERR_INVALID
     assert lib.ovx_corr_plan_create(ctypes.byref(h), 0, 10, 5, 0) == _lib.OVX_ERR_INVALID
+    # real Fftm: even lengths only; axis and direction checked
+    assert lib.ovx_fftm_real_plan_create(ctypes.byref(h), 4, 15, 0, -2, 1.0, 0) == _lib.OVX_ERR_UNSUPPORTED
+    assert lib.ovx_fftm_real_plan_create(ctypes.byref(h), 4, 16, 2, -2, 1.0, 0) == _lib.OVX_ERR_INVALID
+    assert lib.ovx_fftm_real_plan_create(ctypes.byref(h), 4, 16, 0, 0, 1.0, 0) == _lib.OVX_ERR_INVALID
+    assert lib.ovx_fftm_real_out_of_place(None, None, 0, 0, None, 0, 0, 0, 0, None) == _lib.OVX_ERR_INVALID
     # the BF16x3 MFMA variant exists for n = 16384 on device pointers only
     assert lib.ovx_fastconv_plan_create(ctypes.byref(h), 4, 4096, 1.0, _lib.FLAG_BF16X3) == _lib.OVX_ERR_UNSUPPORTED
     assert lib.ovx_fastconv_plan_create(ctypes.byref(h), 4, 16384, 1.0,

@@ -261,3 +261,42 @@ def test_planted_mistakes_are_caught():
     y = o.fastconv64(xx[None, :].astype(np.complex64), w, scale=1.0)  # missing 1/N
     ref_fc = np.fft.ifft(np.fft.fft(xx) * w)
     assert o.rel_l2(y[0], ref_fc) > 1e-2
+
+
+# ---- real <-> complex Fftm ----------------------------------------------------
+
+@pytest.mark.parametrize("n", [2, 4, 6, 16, 30, 64, 1000, 4096])
+def test_real_fftm_oracle_matches_numpy(n):
+    """r2c against numpy.fft.rfft; c2r against n * numpy.fft.irfft (numpy's
+    inverse is normalised, FFTW's c2r is not; both drop the imaginary parts of
+    frequencies 0 and n/2)."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((3, n))
+    xh = o.rfftm64(x)
+    assert xh.shape == (3, n // 2 + 1)
+    assert o.rel_l2(xh, np.fft.rfft(x, axis=1)) < 1e-13
+    y = rng.standard_normal((3, n // 2 + 1)) + 1j * rng.standard_normal((3, n // 2 + 1))
+    assert o.rel_l2(o.irfftm64(y, n), n * np.fft.irfft(y, n, axis=1)) < 1e-13
+    # round trip: c2r(r2c(x)) = n x
+    assert o.rel_l2(o.irfftm64(xh, n), n * x) < 1e-13
+
+
+def test_real_fftm_oracle_closed_forms():
+    n = 32
+    imp = np.zeros((1, n))
+    imp[0, 0] = 1.0
+    assert np.allclose(o.rfftm64(imp), np.ones((1, n // 2 + 1)))      # impulse -> flat
+    const = np.ones((1, n))
+    ref = np.zeros((1, n // 2 + 1), complex)
+    ref[0, 0] = n
+    assert np.allclose(o.rfftm64(const), ref)                         # constant -> DC
+    k0 = 5
+    tone = np.cos(2 * np.pi * k0 * np.arange(n) / n)[None, :]
+    ref = np.zeros((1, n // 2 + 1), complex)
+    ref[0, k0] = n / 2
+    assert np.allclose(o.rfftm64(tone), ref, atol=1e-12)             # cosine -> one bin
+    # a planted mistake (Hermitian extension without the conjugate) fails
+    y = np.random.default_rng(2).standard_normal((1, n // 2 + 1)) * (1 + 1j)
+    bad = o.hermitian_extend(y, n)
+    bad[:, n // 2 + 1:] = np.conj(bad[:, n // 2 + 1:])
+    assert o.rel_l2(o.fftm64(bad, o.INV).real, n * np.fft.irfft(y, n, axis=1)) > 1e-2
