@@ -103,6 +103,30 @@ private:
   plan_holder p_;
 };
 
+// Real Fftm: vsip::Fftm<float, complex<float>, A, fft_fwd> (r2c) and
+// vsip::Fftm<complex<float>, float, A, fft_inv> (c2r) over a rows x cols real
+// domain; the complex side holds n/2 + 1 values per line.
+template <int A, int D>
+class FftmReal {
+public:
+  FftmReal(long rows, long cols, float scale, unsigned flags = 0) : rows_(rows), cols_(cols)
+  {
+    check(ovx_fftm_real_plan_create(p_.out(), rows, cols, A, D, scale, flags));
+  }
+  static bool supports_scale() { return true; }
+  // D == OVX_FFT_FWD: in real, out complex; OVX_FFT_INV: in complex, out real
+  void out_of_place(void const *in, long in_s0, long in_s1, void *out, long out_s0, long out_s1, long rows,
+                    long cols, void *stream = nullptr)
+  {
+    check(ovx_fftm_real_out_of_place(p_.get(), in, in_s0, in_s1, out, out_s0, out_s1, rows, cols, stream));
+  }
+  char const *name() const { return p_.name(); }
+
+private:
+  long rows_, cols_;
+  plan_holder p_;
+};
+
 // vsip::Fft<const_Vector, complex<float>, complex<float>, D> (fft.hpp:75-103).
 template <int D>
 class Fft {

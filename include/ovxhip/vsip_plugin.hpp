@@ -98,7 +98,54 @@ private:
   ovxhip::Fft<dir> plan_;
 };
 
+// fftm_backend<float, complex<float>, A, fft_fwd> (backend.hpp:598-640):
+// real -> complex through the half-length complex kernels.
+template <int A>
+class fftm_r2c : public signal::fft::fftm_backend<float, ctype, A, fft_fwd>
+{
+  static int const axis = A == vsip::col ? OVX_COL : OVX_ROW;
+
+public:
+  fftm_r2c(vsip::Domain<2> const &dom, float scale)
+    : plan_(dom[0].size(), dom[1].size(), scale, OVX_FLAG_HOST_POINTERS)
+  {}
+  virtual char const *name() { return "ovxhip::fftm_r2c"; }
+  virtual bool supports_scale() { return true; }
+  virtual void out_of_place(float *in, stride_type in_s0, stride_type in_s1, ctype *out, stride_type out_s0,
+                            stride_type out_s1, length_type rows, length_type cols)
+  {
+    plan_.out_of_place(in, in_s0, in_s1, out, out_s0, out_s1, rows, cols);
+  }
+
+private:
+  ovxhip::FftmReal<axis, OVX_FFT_FWD> plan_;
+};
+
+// fftm_backend<complex<float>, float, A, fft_inv> (backend.hpp:642-690).
+template <int A>
+class fftm_c2r : public signal::fft::fftm_backend<ctype, float, A, fft_inv>
+{
+  static int const axis = A == vsip::col ? OVX_COL : OVX_ROW;
+
+public:
+  fftm_c2r(vsip::Domain<2> const &dom, float scale)
+    : plan_(dom[0].size(), dom[1].size(), scale, OVX_FLAG_HOST_POINTERS)
+  {}
+  virtual char const *name() { return "ovxhip::fftm_c2r"; }
+  virtual bool supports_scale() { return true; }
+  virtual void out_of_place(ctype *in, stride_type in_s0, stride_type in_s1, float *out, stride_type out_s0,
+                            stride_type out_s1, length_type rows, length_type cols)
+  {
+    plan_.out_of_place(in, in_s0, in_s1, out, out_s0, out_s1, rows, cols);
+  }
+
+private:
+  ovxhip::FftmReal<axis, OVX_FFT_INV> plan_;
+};
+
 inline bool length_ok(length_type n) { return ovx_supported_length((long)n) != 0; }
+// a real transform of even length n runs on the complex kernels of n/2
+inline bool real_length_ok(length_type n) { return n % 2 == 0 && n >= 4 && length_ok(n / 2); }
 
 } // namespace ovxx::ovxhip_backend
 
@@ -120,6 +167,43 @@ struct Evaluator<op::fftm<complex<float>, complex<float>, A, D, R, N>, be::user,
   {
     return std::unique_ptr<signal::fft::fftm_backend<complex<float>, complex<float>, A, D> >(
       new ovxhip_backend::fftm<A, D>(dom, scale));
+  }
+};
+
+// Real Fftm factories (the real-domain size is dom; the transform axis is A).
+template <int A, vsip::return_mechanism_type R, unsigned N>
+struct Evaluator<op::fftm<float, complex<float>, A, fft_fwd, R, N>, be::user,
+                 std::unique_ptr<signal::fft::fftm_backend<float, complex<float>, A, fft_fwd> >(
+                   vsip::Domain<2> const &, float)>
+{
+  static bool const ct_valid = true;
+  static bool rt_valid(vsip::Domain<2> const &dom, float)
+  {
+    return ovxhip_backend::real_length_ok(A == vsip::col ? dom[0].size() : dom[1].size());
+  }
+  static std::unique_ptr<signal::fft::fftm_backend<float, complex<float>, A, fft_fwd> >
+  exec(vsip::Domain<2> const &dom, float scale)
+  {
+    return std::unique_ptr<signal::fft::fftm_backend<float, complex<float>, A, fft_fwd> >(
+      new ovxhip_backend::fftm_r2c<A>(dom, scale));
+  }
+};
+
+template <int A, vsip::return_mechanism_type R, unsigned N>
+struct Evaluator<op::fftm<complex<float>, float, A, fft_inv, R, N>, be::user,
+                 std::unique_ptr<signal::fft::fftm_backend<complex<float>, float, A, fft_inv> >(
+                   vsip::Domain<2> const &, float)>
+{
+  static bool const ct_valid = true;
+  static bool rt_valid(vsip::Domain<2> const &dom, float)
+  {
+    return ovxhip_backend::real_length_ok(A == vsip::col ? dom[0].size() : dom[1].size());
+  }
+  static std::unique_ptr<signal::fft::fftm_backend<complex<float>, float, A, fft_inv> >
+  exec(vsip::Domain<2> const &dom, float scale)
+  {
+    return std::unique_ptr<signal::fft::fftm_backend<complex<float>, float, A, fft_inv> >(
+      new ovxhip_backend::fftm_c2r<A>(dom, scale));
   }
 };
 

@@ -124,7 +124,8 @@ __device__ __noinline__ void tile_ols(lds_job &j, int seg0, lds_ptr lds)
 
 int batch_ols_segments_per_tile(int L) { return kBatchThreads / (L / 16); }
 
-__device__ __forceinline__ FsArgs fs_args_of(lds_job &j)
+template <typename J>
+__device__ __forceinline__ FsArgs fs_args_of(const J &j)
 {
   FsArgs a{};
   a.n1 = j.n1;
@@ -140,82 +141,6 @@ __device__ __forceinline__ FsArgs fs_args_of(lds_job &j)
   a.ldwp = 0;
   a.scale = 1.0f;
   return a;
-}
-
-// pass 1 / 3: column groups; pass 2: line groups
-template <int L, int PASS>
-__device__ __noinline__ void tile_fs(lds_job &j, int row, int group0, lds_ptr lds)
-{
-  using F = FsCfg<L>;
-  static_assert((kBatchThreads / F::NT) * F::LDS <= kBatchLds, "tile LDS");
-  constexpr int G = kBatchThreads / F::NT; // groups per tile
-  const int g = (int)threadIdx.x / F::NT;
-  const int tid = (int)threadIdx.x % F::NT;
-  FsArgs a = fs_args_of(j);
-  if (PASS == 1) {
-    a.in = j.x;
-    a.ld_in = j.ldx;
-  } else {
-    a.in = j.y;
-    a.ld_in = j.ldy;
-  }
-  a.out = j.y;
-  a.ld_out = j.ldy;
-  // the host cuts rows into whole tiles: group0 + G never passes the row end
-  const long group = group0 + g;
-  (void)G;
-  if (PASS == 1) fs_col_body<L, F::P, F::C, 0>(a, row, group, tid, lds + g * F::LDS);
-  else if (PASS == 2) fs_row_body<L, F::P, F::C, 0>(a, row, group, tid, lds + g * F::LDS);
-  else fs_col_body<L, F::P, F::C, 1>(a, row, group, tid, lds + g * F::LDS);
-}
-
-// four-step forward Fftm: pass 1 x -> ws (column FFT + twiddle), pass 2
-// ws -> y (row FFT, LDS-staged natural-order store, scale)
-template <int L, int PASS>
-__device__ __noinline__ void tile_fs_fft(lds_job &j, int row, int group0, lds_ptr lds)
-{
-  using F = FsCfg<L>;
-  static_assert((kBatchThreads / F::NT) * F::LDS <= kBatchLds, "tile LDS");
-  const int g = (int)threadIdx.x / F::NT;
-  const int tid = (int)threadIdx.x % F::NT;
-  FsArgs a = fs_args_of(j);
-  a.scale = j.scale;
-  const long group = group0 + g;
-  if (PASS == 1) {
-    a.in = j.x;
-    a.ld_in = j.ldx;
-    a.out = j.ws;
-    a.ld_out = j.n;
-    fs_col_body<L, F::P, F::C, 0>(a, row, group, tid, lds + g * F::LDS);
-  } else {
-    a.in = j.ws;
-    a.ld_in = j.n;
-    a.out = j.y;
-    a.ld_out = j.ldy;
-    fs_row_body<L, F::P, F::C, 1>(a, row, group, tid, lds + g * F::LDS);
-  }
-}
-
-template <int PASS>
-__device__ __forceinline__ void tile_fs_fft_dispatch(lds_job &j, int row, int group0, lds_ptr lds)
-{
-  const int L = PASS == 2 ? j.n2 : j.n1;
-  switch (L) {
-  case 256: tile_fs_fft<256, PASS>(j, row, group0, lds); break;
-  case 512: tile_fs_fft<512, PASS>(j, row, group0, lds); break;
-  default: break;
-  }
-}
-
-template <int PASS>
-__device__ __forceinline__ void tile_fs_dispatch(lds_job &j, int row, int group0, lds_ptr lds)
-{
-  const int L = PASS == 2 ? j.n2 : j.n1;
-  switch (L) {
-  case 256: tile_fs<256, PASS>(j, row, group0, lds); break;
-  case 512: tile_fs<512, PASS>(j, row, group0, lds); break;
-  default: break;
-  }
 }
 
 __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restrict__ jobs,
@@ -282,8 +207,6 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restr
       default: break;
       }
       break;
-    case BT_FS_FFT_A: tile_fs_fft_dispatch<1>(j, tile.a, tile.b, lds); break;
-    case BT_FS_FFT_B: tile_fs_fft_dispatch<2>(j, tile.a, tile.b, lds); break;
     case BT_OLS:
       switch (j.ols_len) {
       case 1024: tile_ols<1024>(j, tile.a, lds); break;
@@ -294,10 +217,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restr
       default: break;
       }
       break;
-    case BT_FS_A: tile_fs_dispatch<1>(j, tile.a, tile.b, lds); break;
-    case BT_FS_B: tile_fs_dispatch<2>(j, tile.a, tile.b, lds); break;
-    case BT_FS_C: tile_fs_dispatch<3>(j, tile.a, tile.b, lds); break;
-    default: break;
+    default: break; // four-step tiles run in k_batch_fs
     }
     __syncthreads(); // LDS reuse by the next tile
   }
@@ -313,6 +233,89 @@ hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int 
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBatchThreads), 0, s, jobs, tiles, ntiles, counter);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- four-step passes
+// One line group of one row per workgroup (the standalone kernels' shape;
+// four waves per SIMD, the occupancy the LDS of L = 256 groups allows), so
+// co-resident workgroups run independently and overlap their loads with
+// other groups' arithmetic; workgroup w takes tiles w, w + grid, ...  The job
+// descriptor is read from global memory (uniform per tile, scalar loads).
+template <int L>
+__global__ __launch_bounds__(FsCfg<L>::NT, 4) void k_batch_fs(const BatchJob *__restrict__ jobs,
+                                                            const BatchTile *__restrict__ tiles,
+                                                            const int *__restrict__ ntiles)
+{
+  using F = FsCfg<L>;
+  __shared__ float2 lds[F::LDS];
+  const int total = *ntiles;
+  const int tid = (int)threadIdx.x;
+  for (int ti = blockIdx.x; ti < total; ti += gridDim.x) {
+    const BatchTile tile = tiles[ti];
+    const BatchJob &j = jobs[tile.job];
+    FsArgs a = fs_args_of(j);
+    switch (tile.op) {
+    case BT_FS_A: // fastconv pass 1: x -> y, column FFT * twiddle
+      a.in = j.x;
+      a.ld_in = j.ldx;
+      a.out = j.y;
+      a.ld_out = j.ldy;
+      fs_col_body<L, F::P, F::C, 0>(a, tile.a, tile.b, tid, lds);
+      break;
+    case BT_FS_B: // fastconv pass 2 in place on y
+      a.in = j.y;
+      a.ld_in = j.ldy;
+      a.out = j.y;
+      a.ld_out = j.ldy;
+      fs_row_body<L, F::P, F::C, 0>(a, tile.a, tile.b, tid, lds);
+      break;
+    case BT_FS_C: // fastconv pass 3 in place on y: column IFFT
+      a.in = j.y;
+      a.ld_in = j.ldy;
+      a.out = j.y;
+      a.ld_out = j.ldy;
+      fs_col_body<L, F::P, F::C, 1>(a, tile.a, tile.b, tid, lds);
+      break;
+    case BT_FS_FFT_A: // forward Fftm pass 1: x -> workspace
+      a.scale = j.scale;
+      a.in = j.x;
+      a.ld_in = j.ldx;
+      a.out = j.ws;
+      a.ld_out = j.n;
+      fs_col_body<L, F::P, F::C, 0>(a, tile.a, tile.b, tid, lds);
+      break;
+    case BT_FS_FFT_B: // pass 2: workspace -> y in natural order, scaled
+      a.scale = j.scale;
+      a.in = j.ws;
+      a.ld_in = j.n;
+      a.out = j.y;
+      a.ld_out = j.ldy;
+      fs_row_body<L, F::P, F::C, 1>(a, tile.a, tile.b, tid, lds);
+      break;
+    default: break;
+    }
+    __syncthreads(); // LDS reuse by the next tile
+  }
+}
+
+hipError_t launch_batch_fs(int n_line, const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid,
+                           hipStream_t s)
+{
+  if (n_line == 256)
+    hipLaunchKernelGGL(k_batch_fs<256>, dim3(grid), dim3(FsCfg<256>::NT), 0, s, jobs, tiles, ntiles);
+  else if (n_line == 512)
+    hipLaunchKernelGGL(k_batch_fs<512>, dim3(grid), dim3(FsCfg<512>::NT), 0, s, jobs, tiles, ntiles);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// Workgroups of a four-step pass that fit one CU together (LDS-bound).
+int batch_fs_blocks_per_cu(int n_line)
+{
+  if (n_line == 256) return (160 * 1024) / (FsCfg<256>::LDS * 8);
+  if (n_line == 512) return (160 * 1024) / (FsCfg<512>::LDS * 8);
+  return 1;
 }
 
 bool batch_supported(int path, int n)
@@ -332,13 +335,6 @@ int batch_rows_per_tile(int path, int n)
   if (path == BP_HIER) return kBatchThreads / (64 * (n / 1024));
   if (path == BP_ROW) return kBatchThreads / (n / batch_row_p(n));
   return 1;
-}
-
-int batch_groups_per_tile(int n_line)
-{
-  if (n_line == 256) return kBatchThreads / FsCfg<256>::NT;
-  if (n_line == 512) return kBatchThreads / FsCfg<512>::NT;
-  return 0;
 }
 
 int batch_fs_lines_per_group(int n_line)

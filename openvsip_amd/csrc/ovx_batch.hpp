@@ -12,9 +12,9 @@ enum BatchPath { BP_HIER = 1, BP_ROW = 2, BP_FS = 3 };
 enum BatchOp {
   BT_HIER_FASTCONV = 1, // rows [a, a + rows_per_tile) of a hierarchical-path job
   BT_ROW_FASTCONV = 2,  // rows [a, ...) of a register-Stockham-path job
-  BT_FS_A = 3,          // four-step fastconv pass 1: row a, column groups b, b+1, ...
-  BT_FS_B = 4,          // pass 2: row a, line groups b, ...
-  BT_FS_C = 5,          // pass 3: row a, column groups b, ...
+  BT_FS_A = 3,          // four-step fastconv pass 1: row a, column group b
+  BT_FS_B = 4,          // pass 2: row a, line group b
+  BT_FS_C = 5,          // pass 3: row a, column group b
   BT_HIER_FFT_FWD = 6,  // forward Fftm rows of a hierarchical-path job
   BT_ROW_FFT_FWD = 7,   // forward Fftm rows of a register-Stockham-path job
   BT_FS_FFT_A = 8,      // four-step forward Fftm pass 1: x -> workspace
@@ -47,21 +47,35 @@ struct BatchTile {
   int job;
   int op;
   int a; // first row
-  int b; // first group (four-step)
+  int b; // line group (four-step)
 };
+
+// Tile queues of a batch: queue 0 holds the LDS-resident tiles (rows,
+// overlap-save segments) for the persistent launch; the four-step passes get
+// one queue per (pass, line length 256 or 512), run in pass order, one line
+// group per workgroup.
+constexpr int kBatchQueues = 7;
+constexpr int kFsLines[2] = { 256, 512 };
+inline int fs_queue(int pass, int n_line) { return 1 + 2 * pass + (n_line == 512 ? 1 : 0); }
+inline int fs_queue_line(int q) { return kFsLines[(q - 1) % 2]; }
 
 // Threads per persistent workgroup and tile geometry.
 constexpr int kBatchThreads = 1024;
 int batch_rows_per_tile(int path, int n);
 int batch_row_p(int n); // values per thread of a register-Stockham batch tile
 int batch_ols_segments_per_tile(int L);
-int batch_groups_per_tile(int n_line);
 int batch_fs_lines_per_group(int n_line); // C of a four-step group of lines of length n_line
+int batch_fs_blocks_per_cu(int n_line);   // co-resident workgroups of a four-step pass launch
 bool batch_supported(int path, int n);
 
 // One persistent launch over tiles[0 .. *ntiles): `grid` workgroups pull
 // tiles with an atomic counter (which this launch resets first).
 hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int *counter, int grid,
                         hipStream_t s);
+// One four-step pass over tiles[0 .. *ntiles) of line length n_line (256 or
+// 512): `grid` workgroups of one line group each take tiles
+// blockIdx.x, blockIdx.x + grid, ... (no shared counter).
+hipError_t launch_batch_fs(int n_line, const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid,
+                           hipStream_t s);
 
 } // namespace ovx

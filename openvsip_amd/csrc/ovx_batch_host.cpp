@@ -1,6 +1,7 @@
 // ovx_batch_host.cpp -- batches of uneven fastconv jobs (C-ABI part):
-// tile-queue construction, device tables, persistent launches captured once
-// into a hipGraph and replayed, and the plain one-launch-per-job path it is
+// tile-queue construction, device tables, the persistent launch over the
+// LDS-resident tiles and the four-step pass launches captured once into a
+// hipGraph and replayed, and the plain one-launch-per-job path it is
 // compared with.
 #include <hip/hip_runtime.h>
 
@@ -110,37 +111,48 @@ int make_batch_job(const LineFFT &f, long rows, const float2 *x, long ldx, float
   return OVX_OK;
 }
 
-// Tiles of job `ji`: LDS-resident rows go to `lds`, the three four-step
-// passes to pass[0..2].
-void make_batch_tiles(const BatchJob &j, int ji, std::vector<BatchTile> &lds, std::vector<BatchTile> pass[3])
+// Tiles of job `ji`: LDS-resident rows go to queue 0, the three four-step
+// passes to their (pass, line length) queues, one line group per tile.
+void make_batch_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatchQueues])
 {
   if (j.path == BP_HIER || j.path == BP_ROW) {
     int rpt = batch_rows_per_tile(j.path, j.n);
     int op = j.path == BP_HIER ? BT_HIER_FASTCONV : BT_ROW_FASTCONV;
-    for (long r = 0; r < j.rows; r += rpt) lds.push_back(BatchTile{ ji, op, (int)r, 0 });
+    for (long r = 0; r < j.rows; r += rpt) q[0].push_back(BatchTile{ ji, op, (int)r, 0 });
     return;
   }
-  const int gcol = j.n2 / batch_fs_lines_per_group(j.n1), tcol = batch_groups_per_tile(j.n1);
-  const int grow = j.n1 / batch_fs_lines_per_group(j.n2), trow = batch_groups_per_tile(j.n2);
+  const int gcol = j.n2 / batch_fs_lines_per_group(j.n1), grow = j.n1 / batch_fs_lines_per_group(j.n2);
+  std::vector<BatchTile> &a = q[fs_queue(0, j.n1)], &b = q[fs_queue(1, j.n2)], &c = q[fs_queue(2, j.n1)];
   for (long r = 0; r < j.rows; ++r) {
-    for (int g = 0; g < gcol; g += tcol) pass[0].push_back(BatchTile{ ji, BT_FS_A, (int)r, g });
-    for (int g = 0; g < grow; g += trow) pass[1].push_back(BatchTile{ ji, BT_FS_B, (int)r, g });
-    for (int g = 0; g < gcol; g += tcol) pass[2].push_back(BatchTile{ ji, BT_FS_C, (int)r, g });
+    for (int g = 0; g < gcol; ++g) a.push_back(BatchTile{ ji, BT_FS_A, (int)r, g });
+    for (int g = 0; g < grow; ++g) b.push_back(BatchTile{ ji, BT_FS_B, (int)r, g });
+    for (int g = 0; g < gcol; ++g) c.push_back(BatchTile{ ji, BT_FS_C, (int)r, g });
   }
 }
 
-// Spread LDS-resident tiles over the launches so that each launch has about
-// the same number of tiles (every tile is ~16K points of work).
-void balance_queues(std::vector<BatchTile> &lds, std::vector<BatchTile> pass[3], std::vector<BatchTile> q[3])
+void make_batch_fft_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatchQueues])
 {
-  for (int k = 0; k < 3; ++k) q[k] = pass[k];
-  size_t total = lds.size() + pass[0].size() + pass[1].size() + pass[2].size();
-  int nq = pass[0].empty() ? 1 : 3;
-  size_t target = (total + nq - 1) / nq;
-  size_t li = 0;
-  for (int k = 0; k < nq; ++k) {
-    while (li < lds.size() && (q[k].size() < target || k == nq - 1)) q[k].push_back(lds[li++]);
+  const int gcol = j.n2 / batch_fs_lines_per_group(j.n1), grow = j.n1 / batch_fs_lines_per_group(j.n2);
+  std::vector<BatchTile> &a = q[fs_queue(0, j.n1)], &b = q[fs_queue(1, j.n2)];
+  for (long r = 0; r < j.rows; ++r) {
+    for (int g = 0; g < gcol; ++g) a.push_back(BatchTile{ ji, BT_FS_FFT_A, (int)r, g });
+    for (int g = 0; g < grow; ++g) b.push_back(BatchTile{ ji, BT_FS_FFT_B, (int)r, g });
   }
+}
+
+hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[kBatchQueues], int *meta,
+                                  const int *counts, bool all, hipStream_t cap)
+{
+  hipError_t e = hipSuccess;
+  if (all || counts[0] > 0) e = launch_batch(jobs, tiles[0], meta, meta + kMetaCounter, device_cu_count(), cap);
+  for (int k = 1; k < kBatchQueues && e == hipSuccess; ++k) {
+    if (!all && counts[k] == 0) continue;
+    const int line = fs_queue_line(k);
+    int grid = batch_fs_blocks_per_cu(line) * device_cu_count();
+    if (!all) grid = std::min(grid, counts[k]);
+    e = launch_batch_fs(line, jobs, tiles[k], meta + k, grid, cap);
+  }
+  return e;
 }
 
 GraphBatch::~GraphBatch() { release_graph(); }
@@ -153,25 +165,26 @@ void GraphBatch::release_graph()
   graph = nullptr;
 }
 
-// Upload jobs and up to three tile queues; capture the launches once.
-int GraphBatch::build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> q[3], hipStream_t s)
+// Upload jobs and the tile queues; capture the launches once.
+int GraphBatch::build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> q[kBatchQueues], hipStream_t s)
 {
   release_graph();
-  size_t nt = q[0].size() + q[1].size() + q[2].size();
+  size_t nt = 0;
+  for (int k = 0; k < kBatchQueues; ++k) nt += q[k].size();
   int st;
   if ((st = d_jobs.alloc(std::max<size_t>(jobs.size(), 1) * sizeof(BatchJob))) ||
-      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc(8 * sizeof(int))))
+      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc(kMetaInts * sizeof(int))))
     return st;
   std::vector<BatchTile> all;
   all.reserve(nt);
-  int counts[8] = { 0 };
-  nq = 0;
-  for (int k = 0; k < 3; ++k) {
-    if (q[k].empty()) continue;
-    offset[nq] = (int)all.size();
-    counts[nq] = (int)q[k].size();
+  int counts[kMetaInts] = { 0 };
+  BatchTile *tiles[kBatchQueues];
+  nlaunch = 0;
+  for (int k = 0; k < kBatchQueues; ++k) {
+    tiles[k] = (BatchTile *)d_tiles.p + all.size();
+    counts[k] = (int)q[k].size();
     all.insert(all.end(), q[k].begin(), q[k].end());
-    ++nq;
+    if (counts[k]) ++nlaunch;
   }
   hipError_t e = hipSuccess;
   if (!jobs.empty()) e = hipMemcpyAsync(d_jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s);
@@ -181,18 +194,14 @@ int GraphBatch::build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> 
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_fail(e, "batch table upload");
   ntiles = nt;
-  if (nq == 0) return OVX_OK;
+  if (nlaunch == 0) return OVX_OK;
 
   hipStream_t cap;
   e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "capture stream");
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    int *meta = (int *)d_meta.p;
-    const BatchTile *tiles = (const BatchTile *)d_tiles.p;
-    for (int k = 0; k < nq && e == hipSuccess; ++k)
-      e = launch_batch((const BatchJob *)d_jobs.p, tiles + offset[k], meta + k, meta + 4 + k, device_cu_count(),
-                       cap);
+    e = capture_batch_launches((const BatchJob *)d_jobs.p, tiles, (int *)d_meta.p, counts, false, cap);
     hipError_t e2 = hipStreamEndCapture(cap, &graph);
     if (e == hipSuccess) e = e2;
   }
@@ -207,7 +216,7 @@ int GraphBatch::build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> 
 
 int GraphBatch::launch(hipStream_t s) const
 {
-  if (nq == 0) return OVX_OK;
+  if (nlaunch == 0) return OVX_OK;
   hipError_t e = hipGraphLaunch(exec, s);
   if (e != hipSuccess) return hip_fail(e, "batch graph launch");
   return OVX_OK;
@@ -267,7 +276,7 @@ extern "C" int ovx_batch_commit(ovx_batch b, void *stream)
   if (!b) return fail(OVX_ERR_INVALID, "null batch");
   if (b->committed) return fail(OVX_ERR_STATE, "batch already committed");
   hipStream_t s = (hipStream_t)stream;
-  std::vector<BatchTile> lds, pass[3], q[3];
+  std::vector<BatchTile> q[kBatchQueues];
   b->djobs.clear();
   b->paths.clear();
   b->preps.clear();
@@ -290,8 +299,7 @@ extern "C" int ovx_batch_commit(ovx_batch b, void *stream)
   std::vector<int> order(b->djobs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->djobs[a].n > b->djobs[c].n; });
-  for (int i : order) make_batch_tiles(b->djobs[i], i, lds, pass);
-  balance_queues(lds, pass, q);
+  for (int i : order) make_batch_tiles(b->djobs[i], i, q);
   int st = b->gb.build(b->djobs, q, s);
   if (st) return st;
   b->committed = true;
@@ -327,5 +335,5 @@ extern "C" long ovx_batch_tiles(ovx_batch b)
 extern "C" int ovx_batch_launches(ovx_batch b)
 {
   if (!b) return -1;
-  return b->gb.nq;
+  return b->gb.nlaunch;
 }

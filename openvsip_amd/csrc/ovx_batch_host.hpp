@@ -15,23 +15,33 @@ int batch_path_of(const LineFFT &f);
 int device_cu_count();
 int make_batch_job(const LineFFT &f, long rows, const float2 *x, long ldx, float2 *y, long ldy, const float2 *wprep,
                    BatchJob *job);
-void make_batch_tiles(const BatchJob &j, int ji, std::vector<BatchTile> &lds, std::vector<BatchTile> pass[3]);
-void balance_queues(std::vector<BatchTile> &lds, std::vector<BatchTile> pass[3], std::vector<BatchTile> q[3]);
+// Tiles of job `ji` appended to the batch's queues (kBatchQueues of them).
+void make_batch_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatchQueues]);
+// Four-step forward Fftm tiles (x -> workspace -> y) of job `ji`.
+void make_batch_fft_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatchQueues]);
+// Capture the launches of a batch into `graph` on stream `cap`: the
+// persistent launch over queue 0, then one launch per four-step queue in pass
+// order; `all` launches every queue (tile counts read from device memory at
+// run time), otherwise only the non-empty ones in `counts`.
+hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[kBatchQueues], int *meta,
+                                  const int *counts, bool all, hipStream_t cap);
 // Prepared spectrum of a resident replica in the layout of path `f`.
 int replica_prepared(int id, const float2 **prep, long *n, const LineFFT *f);
 
-// Device job/tile tables plus up to three persistent launches captured in
-// one hipGraph.
+// Device job/tile tables plus the batch's launches captured in one hipGraph.
+// meta: tile counts [0 .. kBatchQueues), persistent-launch counter at
+// kMetaCounter.
+constexpr int kMetaInts = 16;
+constexpr int kMetaCounter = 8;
 struct GraphBatch {
-  DevBuf d_jobs, d_tiles, d_meta; // meta: ntiles[0..3], counters[4..7]
+  DevBuf d_jobs, d_tiles, d_meta;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int nq = 0;
-  int offset[3] = { 0, 0, 0 };
+  int nlaunch = 0;
   size_t ntiles = 0;
   ~GraphBatch();
   void release_graph();
-  int build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> q[3], hipStream_t s);
+  int build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> q[kBatchQueues], hipStream_t s);
   int launch(hipStream_t s) const;
 };
 

@@ -142,6 +142,24 @@ __device__ __forceinline__ void stage_coefs(int lane, int wave, const float2 *__
   }
 }
 
+// The coefficient matrices of the inverse direction are the conjugates of
+// the forward ones: negate the Im c entries, i.e. the odd elements of the top
+// operand (-Im c) and the even elements of the bottom operand (Im c), in both
+// the hi and the lo parts (bf16 negation is exact).
+__device__ __forceinline__ void conj_operands(SplitOp &top, SplitOp &bot)
+{
+  u32x4 th = __builtin_bit_cast(u32x4, top.hi), tl = __builtin_bit_cast(u32x4, top.lo);
+  u32x4 bh = __builtin_bit_cast(u32x4, bot.hi), bl = __builtin_bit_cast(u32x4, bot.lo);
+  th ^= 0x80000000u;
+  tl ^= 0x80000000u;
+  bh ^= 0x00008000u;
+  bl ^= 0x00008000u;
+  top.hi = __builtin_bit_cast(bf16x8, th);
+  top.lo = __builtin_bit_cast(bf16x8, tl);
+  bot.hi = __builtin_bit_cast(bf16x8, bh);
+  bot.lo = __builtin_bit_cast(bf16x8, bl);
+}
+
 // Butterfly handled by column c of group gi of this wave in stage S.
 template <int S>
 __device__ __forceinline__ int butterfly(int wave, int gi, int c)
@@ -151,21 +169,16 @@ __device__ __forceinline__ int butterfly(int wave, int gi, int c)
 }
 
 // One radix-16 stage on the LDS row.
+// atop/abot: the stage's coefficient matrix (stage_coefs) as A operands.
 template <int S, int DIR>
-__device__ __forceinline__ void mfma_stage(float2 *lds, const float2 *__restrict__ tw, const float2 *__restrict__ u256,
-                                           int wave, int lane_)
+__device__ __forceinline__ void mfma_stage(float2 *lds, const float2 *__restrict__ tw, const SplitOp &atop,
+                                           const SplitOp &abot, int wave, int lane_)
 {
   constexpr int Ns = E::plan.ns[S];
   constexpr int twoff = E::plan.twoff[S];
   const int lane = fresh(lane_);
   const int c = lane & 15;
   const int h = lane >> 4;
-  SplitOp atop, abot;
-  {
-    float2 cf[4];
-    stage_coefs<S, DIR>(lane, wave, u256, cf);
-    a_operands(cf, atop, abot);
-  }
   f32x4 yr[kGpw], yi[kGpw];
 #pragma unroll
   for (int gi = 0; gi < kGpw; ++gi) {
@@ -225,10 +238,22 @@ __global__ __launch_bounds__(kT) void k_bf16x3_fastconv(const float2 *__restrict
 #pragma unroll
     for (int q = 0; q < 16; ++q) lds[pt + q * slot(kT)] = bload_nt(rx, vt, (uint32_t)(kT * q) * 8u);
   }
+  // forward coefficient matrices: the DFT-16 (stages 0 and 2) and stage 1's
+  // folded matrix; the inverse ones are their conjugates
+  SplitOp ftop, fbot, gtop, gbot;
+  {
+    float2 cf[4];
+    stage_coefs<0, -1>(lane, wave, u256, cf);
+    a_operands(cf, ftop, fbot);
+    stage_coefs<1, -1>(lane, wave, u256, cf);
+    a_operands(cf, gtop, gbot);
+  }
   __syncthreads();
-  mfma_stage<0, -1>(lds, tw, u256, wave, lane);
-  mfma_stage<1, -1>(lds, tw, u256, wave, lane);
-  mfma_stage<2, -1>(lds, tw, u256, wave, lane);
+  mfma_stage<0, -1>(lds, tw, ftop, fbot, wave, lane);
+  mfma_stage<1, -1>(lds, tw, gtop, gbot, wave, lane);
+  mfma_stage<2, -1>(lds, tw, ftop, fbot, wave, lane);
+  conj_operands(ftop, fbot);
+  conj_operands(gtop, gbot);
   // radix-4 stage (FP32, VALU) in slot form, spectrum multiply, back in place
   {
     float2 v[16];
@@ -242,9 +267,9 @@ __global__ __launch_bounds__(kT) void k_bf16x3_fastconv(const float2 *__restrict
       lds[pt + q * slot(kT)] = cscale(cmul(v[q], bload(rw, vt, (uint32_t)(kT * q) * 8u)), scale);
   }
   __syncthreads();
-  mfma_stage<0, +1>(lds, tw, u256, wave, lane);
-  mfma_stage<1, +1>(lds, tw, u256, wave, lane);
-  mfma_stage<2, +1>(lds, tw, u256, wave, lane);
+  mfma_stage<0, +1>(lds, tw, ftop, fbot, wave, lane);
+  mfma_stage<1, +1>(lds, tw, gtop, gbot, wave, lane);
+  mfma_stage<2, +1>(lds, tw, ftop, fbot, wave, lane);
   {
     float2 v[16];
     const int ts = fresh(t);

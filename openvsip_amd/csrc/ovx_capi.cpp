@@ -509,11 +509,11 @@ extern "C" int ovx_fftm_out_of_place(ovx_plan p, const void *in, long in_s0, lon
 {
   int st = check_plan(p, PLAN_FFTM);
   if (st) return st;
-  if (!in || !out) return fail(OVX_ERR_INVALID, "null data pointer");
   long n, lines, ldi, ldo;
   if ((st = fftm_geometry(p, in_s0, in_s1, rows, cols, &n, &lines, &ldi))) return st;
   if ((st = fftm_geometry(p, out_s0, out_s1, rows, cols, &n, &lines, &ldo))) return st;
-  if (n == 0 || lines == 0) return OVX_OK;
+  if (n == 0 || lines == 0) return OVX_OK; // empty views may carry null pointers
+  if (!in || !out) return fail(OVX_ERR_INVALID, "null data pointer");
   hipStream_t s = (hipStream_t)stream;
   int dir = sign_of(p->dir);
   if (p->flags & OVX_FLAG_HOST_POINTERS) {
@@ -613,7 +613,6 @@ extern "C" int ovx_fftm_real_out_of_place(ovx_plan p, const void *in, long in_s0
 {
   int st = check_plan(p, PLAN_FFTM_REAL);
   if (st) return st;
-  if (!in || !out) return fail(OVX_ERR_INVALID, "null data pointer");
   if (rows != p->rows || cols != p->cols)
     return fail(OVX_ERR_INVALID, "size %ldx%ld does not match the plan's %ldx%ld", rows, cols, p->rows, p->cols);
   const bool fwd = p->dir == OVX_FFT_FWD;
@@ -622,7 +621,8 @@ extern "C" int ovx_fftm_real_out_of_place(ovx_plan p, const void *in, long in_s0
   // real side: n floats per line; complex side: h + 1 values per line
   if ((st = real_side(p, fwd ? in_s0 : out_s0, fwd ? in_s1 : out_s1, n, &lines, &ldr))) return st;
   if ((st = real_side(p, fwd ? out_s0 : in_s0, fwd ? out_s1 : in_s1, h + 1, &lines, &ldc))) return st;
-  if (n == 0 || lines == 0) return OVX_OK;
+  if (n == 0 || lines == 0) return OVX_OK; // empty views may carry null pointers
+  if (!in || !out) return fail(OVX_ERR_INVALID, "null data pointer");
   if (lines > 1 && (ldr % 2))
     return fail(OVX_ERR_UNSUPPORTED, "real line stride %ld must be even (lines are read as complex pairs)", ldr);
   const void *real_ptr = fwd ? in : out;
@@ -689,12 +689,12 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
 {
   int st = check_plan(p, PLAN_FASTCONV);
   if (st) return st;
-  if (!x || !w || !y) return fail(OVX_ERR_INVALID, "null data pointer");
   if (rows < 0 || rows > p->rows) return fail(OVX_ERR_INVALID, "rows %ld outside the plan's %ld", rows, p->rows);
   long n = p->cols;
   if (rows > 1 && (ldx < n || ldy < n)) return fail(OVX_ERR_INVALID, "row stride shorter than n");
   if (ldw != 0 && rows > 1 && ldw < n) return fail(OVX_ERR_INVALID, "coefficient row stride shorter than n");
-  if (rows == 0) return OVX_OK;
+  if (rows == 0) return OVX_OK; // empty views may carry null pointers
+  if (!x || !w || !y) return fail(OVX_ERR_INVALID, "null data pointer");
   hipStream_t s = (hipStream_t)stream;
   if (p->bf16x3) {
     if (ldw != 0) return fail(OVX_ERR_UNSUPPORTED, "BF16x3 variant takes one spectrum vector");

@@ -18,11 +18,10 @@
 //
 // LDS layout of a line (LdsLayout): gfx950's LDS has 64 banks of 4 bytes and
 // serves a 64-bit access for 32 lanes per cycle, so the 32 lanes of each
-// half-wave must hit 32 distinct 8-byte slots.  Lines handled by at least 32
-// threads use the XOR swizzle slot = pos ^ ((pos >> 4) & 31), a permutation
-// inside every aligned block of 32 elements that keeps the Stockham write
-// strides (1, R, Ns) and the slot-form reads on distinct slots; shorter lines
-// (several per workgroup) use one pad element every 16.
+// half-wave should hit 32 distinct 8-byte slots.  Lines handled by at least
+// 16 threads insert one pad element every 32 (slot = pos + pos/32), shorter
+// ones one every 16; both keep the slot-form reads t + T q at constant
+// offsets from pos(t), so the exchanges need no per-access address math.
 #pragma once
 #include "ovx_device.hpp"
 
@@ -73,12 +72,9 @@ constexpr int lds_row_elems(int N) { return N >= 16 ? N + N / 16 : N; }
 
 template <int N, int P>
 struct LdsLayout {
-  static constexpr bool kSwizzle = (N / P) >= 32 && N % 32 == 0;
-  static constexpr int elems = kSwizzle ? N : lds_row_elems(N);
-  __host__ __device__ static constexpr int idx(int pos)
-  {
-    return kSwizzle ? (pos ^ ((pos >> 4) & 31)) : pos + (pos >> 4);
-  }
+  static constexpr bool kPad32 = (N / P) >= 16 && N >= 64;
+  static constexpr int elems = kPad32 ? N + (N + 31) / 32 : lds_row_elems(N);
+  __host__ __device__ static constexpr int idx(int pos) { return kPad32 ? pos + (pos >> 5) : pos + (pos >> 4); }
 };
 
 // Pointer into LDS with the address space spelled out (for code that is not

@@ -24,8 +24,8 @@ struct Hier {
 
 // Outer forward stage: load, DFT_R0, twiddle, write slices.
 template <int N, typename LP>
-__device__ __forceinline__ void outer_forward(const rsrc_t &rx, uint32_t vx, bool valid, LP lds,
-                                              const float2 *__restrict__ two, int t)
+__device__ __forceinline__ void outer_forward(const rsrc_t &rx, uint32_t vx, bool valid, LP lds, const rsrc_t &rto,
+                                              int t)
 {
   using H = Hier<N>;
   constexpr int R0 = H::R0, M0 = H::M0, NW = H::NW;
@@ -39,7 +39,7 @@ __device__ __forceinline__ void outer_forward(const rsrc_t &rx, uint32_t vx, boo
     if constexpr (R0 > 1) {
       Dft<R0, -1>::run(a);
 #pragma unroll
-      for (int k1 = 1; k1 < R0; ++k1) a[k1] = cmul(a[k1], two[(k1 - 1) * NW + j]);
+      for (int k1 = 1; k1 < R0; ++k1) a[k1] = cmul(a[k1], bload(rto, (uint32_t)j * 8u, (uint32_t)((k1 - 1) * NW) * 8u));
     }
 #pragma unroll
     for (int k1 = 0; k1 < R0; ++k1) lds_st(lds, k1 * H::LSL + H::WE::Layout::idx(j), a[k1]);
@@ -90,11 +90,12 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
   const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
   const rsrc_t rw = make_rsrc(a.w + row0 * a.ldw, 0xffffffffu);
   const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
+  const rsrc_t rto = make_rsrc(a.tw_outer, (uint32_t)((R0 - 1) * H::NW) * 8u); // outer twiddles
   const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
   const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
 
   if constexpr (R0 > 1) {
-    outer_forward<N, LP>(rx, vx, valid, lds, a.tw_outer, t);
+    outer_forward<N, LP>(rx, vx, valid, lds, rto, t);
     __syncthreads();
   }
   // ---- wave stage: wave k1 owns slice k1
@@ -134,7 +135,8 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
   if constexpr (R0 > 1) {
     if (k1 > 0) {
 #pragma unroll
-      for (int s = 0; s < 16; ++s) u[s] = cmulc(u[s], a.tw_outer[(k1 - 1) * NW + l + 64 * s]);
+      for (int s = 0; s < 16; ++s)
+        u[s] = cmulc(u[s], bload(rto, (uint32_t)((k1 - 1) * NW + l) * 8u, (uint32_t)(64 * s) * 8u));
     }
 #pragma unroll
     for (int s = 0; s < 16; ++s) lds_st(sl, H::WE::Layout::idx(l + 64 * s), u[s]);
@@ -161,11 +163,12 @@ __device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int 
   const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
   const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
   const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
+  const rsrc_t rto = make_rsrc(a.tw_outer, (uint32_t)((R0 - 1) * H::NW) * 8u); // outer twiddles
   const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
   const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
 
   if constexpr (R0 > 1) {
-    outer_forward<N, LP>(rx, vx, valid, lds, a.tw_outer, t);
+    outer_forward<N, LP>(rx, vx, valid, lds, rto, t);
     __syncthreads();
   }
   const int k1 = t >> 6;
@@ -214,6 +217,7 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
   const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
   const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
   const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
+  const rsrc_t rto = make_rsrc(a.tw_outer, (uint32_t)((R0 - 1) * H::NW) * 8u); // outer twiddles
   const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
   const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
   const int k1 = t >> 6;
@@ -237,7 +241,8 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
   if constexpr (R0 > 1) {
     if (k1 > 0) {
 #pragma unroll
-      for (int s = 0; s < 16; ++s) u[s] = cmulc(u[s], a.tw_outer[(k1 - 1) * NW + l + 64 * s]);
+      for (int s = 0; s < 16; ++s)
+        u[s] = cmulc(u[s], bload(rto, (uint32_t)((k1 - 1) * NW + l) * 8u, (uint32_t)(64 * s) * 8u));
     }
 #pragma unroll
     for (int s = 0; s < 16; ++s) lds_st(sl, H::WE::Layout::idx(l + 64 * s), u[s]);

@@ -98,9 +98,9 @@ struct Req {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  DevBuf d_jobs, d_tiles[3], d_meta, ws;
+  DevBuf d_jobs, d_tiles[kBatchQueues], d_meta, ws;
   BatchJob *h_jobs = nullptr;
-  BatchTile *h_tiles[3] = { nullptr, nullptr, nullptr };
+  BatchTile *h_tiles[kBatchQueues] = {};
   int *h_meta = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -155,7 +155,7 @@ Server::~Server()
     if (s->done) (void)hipEventDestroy(s->done);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     free_pinned(s->h_jobs);
-    for (int k = 0; k < 3; ++k) free_pinned(s->h_tiles[k]);
+    for (int k = 0; k < kBatchQueues; ++k) free_pinned(s->h_tiles[k]);
     free_pinned(s->h_meta);
   }
 }
@@ -167,26 +167,27 @@ int Server::build_slot(Slot &s)
   if (e != hipSuccess) return hip_fail(e, "slot stream");
   int st;
   size_t jb = (size_t)cfg.max_jobs * sizeof(BatchJob), tb = (size_t)max_tiles * sizeof(BatchTile);
-  if ((st = s.d_jobs.alloc(jb)) || (st = s.d_meta.alloc(8 * sizeof(int)))) return st;
-  for (int k = 0; k < 3; ++k)
+  if ((st = s.d_jobs.alloc(jb)) || (st = s.d_meta.alloc(kMetaInts * sizeof(int)))) return st;
+  for (int k = 0; k < kBatchQueues; ++k)
     if ((st = s.d_tiles[k].alloc(tb))) return st;
   if (cfg.workspace_mb > 0 && (st = s.ws.alloc((size_t)cfg.workspace_mb << 20))) return st;
   e = hipHostMalloc((void **)&s.h_jobs, jb, hipHostMallocDefault);
-  for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipHostMalloc((void **)&s.h_tiles[k], tb, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipHostMalloc((void **)&s.h_meta, 8 * sizeof(int), hipHostMallocDefault);
+  for (int k = 0; k < kBatchQueues && e == hipSuccess; ++k)
+    e = hipHostMalloc((void **)&s.h_tiles[k], tb, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&s.h_meta, kMetaInts * sizeof(int), hipHostMallocDefault);
   if (e != hipSuccess) return hip_fail(e, "pinned staging");
   if (cfg.plain) return OVX_OK;
-  // capture the three persistent launches once; every batch reuses them with
-  // new table contents (tile counts are read from device memory)
+  // capture the launches (the persistent one and one per four-step queue)
+  // once; every batch reuses them with new table contents (tile counts are
+  // read from device memory, an empty queue's launch exits at once)
   hipStream_t cap;
   e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "capture stream");
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    int *meta = (int *)s.d_meta.p;
-    for (int k = 0; k < 3 && e == hipSuccess; ++k)
-      e = launch_batch((const BatchJob *)s.d_jobs.p, (const BatchTile *)s.d_tiles[k].p, meta + k, meta + 4 + k,
-                       device_cu_count(), cap);
+    BatchTile *tiles[kBatchQueues];
+    for (int k = 0; k < kBatchQueues; ++k) tiles[k] = (BatchTile *)s.d_tiles[k].p;
+    e = capture_batch_launches((const BatchJob *)s.d_jobs.p, tiles, (int *)s.d_meta.p, nullptr, true, cap);
     hipError_t e2 = hipStreamEndCapture(cap, &s.graph);
     if (e == hipSuccess) e = e2;
   }
@@ -224,10 +225,10 @@ static long tiles_of(const Req &q, const LineFFT *f, const CorrRef *cr)
   if (r.kind == OVX_REQ_CORR) return (corr_segments(*cr) + batch_ols_segments_per_tile(cr->L) - 1) /
                                        batch_ols_segments_per_tile(cr->L);
   int path = batch_path_of(*f);
-  if (path == BP_FS) {
-    long gc = f->n2 / batch_fs_lines_per_group((int)f->n1) / batch_groups_per_tile((int)f->n1);
-    long gr = f->n1 / batch_fs_lines_per_group((int)f->n2) / batch_groups_per_tile((int)f->n2);
-    return r.rows * (2 * gc + gr);
+  if (path == BP_FS) { // one line group per tile, per pass
+    long gc = f->n2 / batch_fs_lines_per_group((int)f->n1);
+    long gr = f->n1 / batch_fs_lines_per_group((int)f->n2);
+    return r.rows * std::max(gc, gr);
   }
   long rpt = batch_rows_per_tile(path, (int)r.n);
   return (r.rows + rpt - 1) / rpt;
@@ -236,7 +237,7 @@ static long tiles_of(const Req &q, const LineFFT *f, const CorrRef *cr)
 int Server::admit_and_launch(Slot &s, std::vector<Req *> &batch)
 {
   std::vector<BatchJob> jobs;
-  std::vector<BatchTile> lds, pass[3], q[3];
+  std::vector<BatchTile> q[kBatchQueues];
   size_t ws_off = 0;
   for (size_t i = 0; i < batch.size(); ++i) {
     const ovx_request &r = batch[i]->r;
@@ -252,7 +253,7 @@ int Server::admit_and_launch(Slot &s, std::vector<Req *> &batch)
       jobs.push_back(j);
       int spt = batch_ols_segments_per_tile(cr->L);
       for (long sgm = 0; sgm < corr_segments(*cr); sgm += spt)
-        lds.push_back(BatchTile{ (int)i, BT_OLS, (int)sgm, 0 });
+        q[0].push_back(BatchTile{ (int)i, BT_OLS, (int)sgm, 0 });
       continue;
     }
     const LineFFT *f = shared_path(r.n, &st);
@@ -263,43 +264,38 @@ int Server::admit_and_launch(Slot &s, std::vector<Req *> &batch)
       if ((st = replica_prepared(r.ref, &prep, &rn, f))) return st;
       if ((st = make_batch_job(*f, r.rows, (const float2 *)r.x, r.ldx, (float2 *)r.y, r.ldy, prep, &j))) return st;
       jobs.push_back(j);
-      make_batch_tiles(j, (int)i, lds, pass);
+      make_batch_tiles(j, (int)i, q);
     } else { // forward Fftm
       if ((st = make_batch_job(*f, r.rows, (const float2 *)r.x, r.ldx, (float2 *)r.y, r.ldy, nullptr, &j))) return st;
       j.scale = 1.0f;
       if (j.path == BP_FS) {
         j.ws = (float2 *)((char *)s.ws.p + ws_off);
         ws_off += batch[i]->ws_need;
-        const int gcol = j.n2 / batch_fs_lines_per_group(j.n1), tcol = batch_groups_per_tile(j.n1);
-        const int grow = j.n1 / batch_fs_lines_per_group(j.n2), trow = batch_groups_per_tile(j.n2);
-        for (long rr = 0; rr < j.rows; ++rr) {
-          for (int g = 0; g < gcol; g += tcol) pass[0].push_back(BatchTile{ (int)i, BT_FS_FFT_A, (int)rr, g });
-          for (int g = 0; g < grow; g += trow) pass[1].push_back(BatchTile{ (int)i, BT_FS_FFT_B, (int)rr, g });
-        }
+        make_batch_fft_tiles(j, (int)i, q);
       } else {
         int rpt = batch_rows_per_tile(j.path, j.n);
         int op = j.path == BP_HIER ? BT_HIER_FFT_FWD : BT_ROW_FFT_FWD;
-        for (long rr = 0; rr < j.rows; rr += rpt) lds.push_back(BatchTile{ (int)i, op, (int)rr, 0 });
+        for (long rr = 0; rr < j.rows; rr += rpt) q[0].push_back(BatchTile{ (int)i, op, (int)rr, 0 });
       }
       jobs.push_back(j);
     }
   }
-  balance_queues(lds, pass, q);
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < kBatchQueues; ++k)
     if ((long)q[k].size() > max_tiles) return fail(OVX_ERR_STATE, "slot tile table overflow");
   std::copy(jobs.begin(), jobs.end(), s.h_jobs);
-  int meta[8] = { 0 };
-  for (int k = 0; k < 3; ++k) {
+  int meta[kMetaInts] = { 0 };
+  for (int k = 0; k < kBatchQueues; ++k) {
     std::copy(q[k].begin(), q[k].end(), s.h_tiles[k]);
     meta[k] = (int)q[k].size();
   }
   std::copy(meta, meta + 8, s.h_meta);
   hipError_t e = hipMemcpyAsync(s.d_jobs.p, s.h_jobs, jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s.stream);
-  for (int k = 0; k < 3 && e == hipSuccess; ++k)
+  for (int k = 0; k < kBatchQueues && e == hipSuccess; ++k)
     if (!q[k].empty())
       e = hipMemcpyAsync(s.d_tiles[k].p, s.h_tiles[k], q[k].size() * sizeof(BatchTile), hipMemcpyHostToDevice,
                          s.stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(s.d_meta.p, s.h_meta, 8 * sizeof(int), hipMemcpyHostToDevice, s.stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s.d_meta.p, s.h_meta, kMetaInts * sizeof(int), hipMemcpyHostToDevice, s.stream);
   if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
   if (e != hipSuccess) return hip_fail(e, "batch launch");
   return OVX_OK;

@@ -46,7 +46,8 @@ def test_batch_matches_oracle_and_standalone(ovx):
         b.add(xd, reps[n][0], yd)
     b.commit()
     print("tiles", b.tiles, "launches", b.launches, flush=True)
-    assert b.launches == 3  # a four-step job is in the mix
+    # persistent launch + one launch per four-step pass (line length 256)
+    assert b.launches == 4
     b()
     outs = [host(yd) for *_, yd in jobs]
     for (n, rows, x, xd, yd), y in zip(jobs, outs):
@@ -84,3 +85,21 @@ def test_batch_lds_only_single_launch(ovx):
     b()
     for n, rows, x, xd, yd in jobs:
         assert o.rel_l2(host(yd), o.fastconv64(x, reps[n][1])) < 1e-5
+
+
+def test_batch_four_step_only_and_mixed_line_lengths(ovx):
+    """Four-step jobs alone (no persistent launch), including 131072 = 256 x
+    512, whose middle pass runs lines of 512: one launch per non-empty
+    (pass, line length) queue."""
+    jobs, reps = make_jobs(ovx, [(65536, 3), (131072, 2)], seed=41)
+    b = ovx.Batch()
+    for n, rows, x, xd, yd in jobs:
+        b.add(xd, reps[n][0], yd)
+    b.commit()
+    assert b.launches == 4  # pass 1 (256), pass 2 (256 and 512), pass 3 (256)
+    b()
+    for n, rows, x, xd, yd in jobs:
+        y = host(yd)
+        assert o.rel_l2(y, o.fastconv64(x, reps[n][1])) < 1e-5, n
+        alone = ovx.Fastconv(rows, n).with_replica(xd, reps[n][0])
+        assert np.array_equal(host(alone), y), n
