@@ -96,19 +96,20 @@ def cpu_baseline(x, w, seconds):
 
 
 def measured_traffic(plan_name, rows, n):
-    """HBM bytes per launch from the committed rocprof counter summary of this
-    kernel at this shape (FETCH_SIZE and WRITE_SIZE, each calibrated against
-    a device copy), or None when none was recorded."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"k_hier_fastconv_{n}x{rows}_hbm_traffic.json")
-    try:
-        with open(path) as f:
-            rec = json.load(f)
-    except OSError:
-        return None
-    if rec.get("plan_name") != plan_name or rec.get("rows") != rows or rec.get("n") != n:
-        return None
-    return rec["traffic_bytes"]
+    """HBM bytes per launch from the committed rocprof counter summaries
+    (profiles/*_hbm_traffic.json: FETCH_SIZE and WRITE_SIZE of this kernel at
+    this shape, each calibrated against a device copy), or None when none was
+    recorded."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.json"))):
+        try:
+            with open(path) as f:
+                rec = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if rec.get("plan_name") == plan_name and rec.get("rows") == rows and rec.get("n") == n:
+            return rec["traffic_bytes"]
+    return None
 
 
 def precision_section(args, x, w, xd, fp32_ms, fp32_rel):

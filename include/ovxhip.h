@@ -132,9 +132,9 @@ int ovx_fftm_in_place(ovx_plan plan, void *inout, long s0, long s1, long rows,
 /* Plan for a rows x cols *real* domain, transforms of even length n = cols
    (OVX_ROW) or rows (OVX_COL); the complex side holds n/2 + 1 values per line.
    OVX_FFT_FWD: real -> complex, replaces fftm_backend<T, complex<T>, A,
-   fft_fwd> (backend.hpp:598-640; FFTW r2c in fftw.cpp:778-843).
+   fft_fwd> (backend.hpp:600-646; FFTW r2c in fftw.cpp:778-843).
    OVX_FFT_INV: complex -> real, replaces fftm_backend<complex<T>, T, A,
-   fft_inv> (backend.hpp:642-690; FFTW c2r in fftw.cpp:845-903): unnormalised
+   fft_inv> (backend.hpp:648-694; FFTW c2r in fftw.cpp:845-903): unnormalised
    like FFTW's c2r, the imaginary parts of frequencies 0 and n/2 ignored.
    The result is multiplied by `scale` in the kernel. */
 int ovx_fftm_real_plan_create(ovx_plan *plan, long rows, long cols, int axis,

@@ -98,7 +98,7 @@ private:
   ovxhip::Fft<dir> plan_;
 };
 
-// fftm_backend<float, complex<float>, A, fft_fwd> (backend.hpp:598-640):
+// fftm_backend<float, complex<float>, A, fft_fwd> (backend.hpp:600-646):
 // real -> complex through the half-length complex kernels.
 template <int A>
 class fftm_r2c : public signal::fft::fftm_backend<float, ctype, A, fft_fwd>
@@ -121,7 +121,7 @@ private:
   ovxhip::FftmReal<axis, OVX_FFT_FWD> plan_;
 };
 
-// fftm_backend<complex<float>, float, A, fft_inv> (backend.hpp:642-690).
+// fftm_backend<complex<float>, float, A, fft_inv> (backend.hpp:648-694).
 template <int A>
 class fftm_c2r : public signal::fft::fftm_backend<ctype, float, A, fft_inv>
 {

@@ -191,6 +191,8 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restr
       switch (j.n) {
       case 256: tile_row_fastconv<256, 16>(j, tile.a, lds); break;
       case 512: tile_row_fastconv<512, 16>(j, tile.a, lds); break;
+      case 2048: tile_row_fastconv<2048, 16>(j, tile.a, lds); break;
+      case 4096: tile_row_fastconv<4096, 16>(j, tile.a, lds); break;
       case 3072: tile_row_fastconv<3072, 12>(j, tile.a, lds); break;
       case 6144: tile_row_fastconv<6144, 12>(j, tile.a, lds); break;
       case 12288: tile_row_fastconv<12288, 12>(j, tile.a, lds); break;
@@ -201,6 +203,8 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restr
       switch (j.n) {
       case 256: tile_row_fwd<256, 16>(j, tile.a, lds); break;
       case 512: tile_row_fwd<512, 16>(j, tile.a, lds); break;
+      case 2048: tile_row_fwd<2048, 16>(j, tile.a, lds); break;
+      case 4096: tile_row_fwd<4096, 16>(j, tile.a, lds); break;
       case 3072: tile_row_fwd<3072, 12>(j, tile.a, lds); break;
       case 6144: tile_row_fwd<6144, 12>(j, tile.a, lds); break;
       case 12288: tile_row_fwd<12288, 12>(j, tile.a, lds); break;
@@ -321,14 +325,15 @@ int batch_fs_blocks_per_cu(int n_line)
 bool batch_supported(int path, int n)
 {
   if (path == BP_HIER) return hier_supported(n);
-  if (path == BP_ROW) return n == 256 || n == 512 || n == 3072 || n == 6144 || n == 12288;
+  if (path == BP_ROW) return n == 256 || n == 512 || n == 2048 || n == 4096 || n == 3072 || n == 6144 || n == 12288;
   if (path == BP_FS) return n == 65536 || n == 131072 || n == 262144;
   return false;
 }
 
 // Batch tiles of the 3 * 2^k lengths use 12 values per thread, so that 1024
-// threads fit the 128-VGPR budget of a 1024-thread workgroup.
-int batch_row_p(int n) { return (n == 256 || n == 512) ? 16 : 12; }
+// threads fit the 128-VGPR budget of a 1024-thread workgroup; the powers of
+// two use 16.
+int batch_row_p(int n) { return (n == 256 || n == 512 || n == 2048 || n == 4096) ? 16 : 12; }
 
 int batch_rows_per_tile(int path, int n)
 {

@@ -27,7 +27,7 @@ static std::map<long, std::unique_ptr<LineFFT>> g_paths;
 // Variant the batch tiles of length n run: the 3 * 2^k lengths use the
 // 12-values-per-thread register-Stockham configuration (variant 1), which
 // fits the persistent kernel's 1024-thread register budget.
-int batch_variant(long n) { return (n == 3072 || n == 6144 || n == 12288) ? 1 : 0; }
+int batch_variant(long n) { return (n == 3072 || n == 6144 || n == 12288) ? 1 : LineFFT::default_variant(n); }
 
 const LineFFT *shared_path(long n, int *status)
 {
@@ -148,8 +148,10 @@ hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[k
   for (int k = 1; k < kBatchQueues && e == hipSuccess; ++k) {
     if (!all && counts[k] == 0) continue;
     const int line = fs_queue_line(k);
-    int grid = batch_fs_blocks_per_cu(line) * device_cu_count();
-    if (!all) grid = std::min(grid, counts[k]);
+    // a batch with known counts launches one workgroup per tile (as the
+    // standalone passes do); the server's graph, whose counts change per
+    // batch, strides a resident grid over the queue
+    const int grid = all ? batch_fs_blocks_per_cu(line) * device_cu_count() : counts[k];
     e = launch_batch_fs(line, jobs, tiles[k], meta + k, grid, cap);
   }
   return e;

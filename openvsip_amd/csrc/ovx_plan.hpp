@@ -61,6 +61,7 @@ struct LineFFT {
   int fastconv_prepared(const float2 *x, long ldx, const float2 *wp, float2 *y, long ldy, long lines,
                         hipStream_t s) const;
   static int variants(long n);
+  static int default_variant(long n);
   // every table the kernels of this path read has been allocated
   bool tables_ready() const;
 };

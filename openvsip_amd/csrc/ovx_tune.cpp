@@ -5,7 +5,7 @@
 // different values per thread) gives the same result to rounding; which is
 // fastest depends on the length and on how many rows share the GPU.  The
 // tuner times each variant's fused fastconv on scratch data of the requested
-// shape (three interleaved rounds, best round per variant) and records the
+// shape (five interleaved rounds, best round per variant) and records the
 // winner under (n, rows rounded up to a power of two).
 // Plans created with the default variant flag consult the cache.  The cache
 // is a text file, one "n rows_bucket variant microseconds" line per entry,
@@ -93,7 +93,7 @@ static int time_variant(long n, long rows, int variant, float2 *x, float2 *w, fl
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return fail(OVX_ERR_DEVICE, "event create");
   for (int i = 0; i < 3 && !st; ++i) st = f.fastconv(x, n, w, 0, y, n, rows, 1.0f / (float)n, s);
-  const int iters = 20;
+  const int iters = 50;
   hipError_t e = hipEventRecord(e0, s);
   for (int i = 0; i < iters && !st; ++i) st = f.fastconv(x, n, w, 0, y, n, rows, 1.0f / (float)n, s);
   if (e == hipSuccess) e = hipEventRecord(e1, s);
@@ -126,10 +126,10 @@ extern "C" int ovx_autotune_fastconv(long n, long rows, int *best_variant, doubl
   if (e != hipSuccess) return hip_fail(e, "tuning buffers");
   hipStream_t s;
   if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "tuning stream");
-  // three interleaved rounds, the fastest round of each variant counts
+  // five interleaved rounds, the fastest round of each variant counts
   // (robust to clock ramp-up and to a busy device during one round)
   std::vector<double> t_min(nv, 1e30);
-  for (int round = 0; round < 3 && (!st || st == OVX_ERR_UNSUPPORTED); ++round) {
+  for (int round = 0; round < 5 && (!st || st == OVX_ERR_UNSUPPORTED); ++round) {
     for (int v = 0; v < nv; ++v) {
       double us;
       st = time_variant(n, rows, v, (float2 *)x.p, (float2 *)w.p, (float2 *)y.p, s, &us);

@@ -56,8 +56,9 @@ def test_batch_matches_oracle_and_standalone(ovx):
         err = o.rel_l2(y, o.fastconv64(x, reps[n][1]))
         assert err < 1e-5, (n, rows, err)
         # the batch runs the 3 * 2^k lengths with 12 values per thread
-        # (variant 1 of those lengths); every other length uses the default
-        variant = 1 if n in (3072, 6144, 12288) else None
+        # (variant 1 of those lengths) and 2048 / 4096 on the register-Stockham
+        # kernel (variant 1); every other length on variant 0
+        variant = 1 if n in (3072, 6144, 12288, 2048, 4096) else 0
         alone = ovx.Fastconv(rows, n, variant=variant).with_replica(xd, reps[n][0])
         assert np.array_equal(host(alone), y), (n, rows)
     # replay: same results again (graph replays are independent)
