@@ -245,10 +245,10 @@ hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int 
 // co-resident workgroups run independently and overlap their loads with
 // other groups' arithmetic; workgroup w takes tiles w, w + grid, ...  The job
 // descriptor is read from global memory (uniform per tile, scalar loads).
-template <int L>
+template <int L, int PASS>
 __global__ __launch_bounds__(FsCfg<L>::NT, 4) void k_batch_fs(const BatchJob *__restrict__ jobs,
-                                                            const BatchTile *__restrict__ tiles,
-                                                            const int *__restrict__ ntiles)
+                                                               const BatchTile *__restrict__ tiles,
+                                                               const int *__restrict__ ntiles)
 {
   using F = FsCfg<L>;
   __shared__ float2 lds[F::LDS];
@@ -258,60 +258,58 @@ __global__ __launch_bounds__(FsCfg<L>::NT, 4) void k_batch_fs(const BatchJob *__
     const BatchTile tile = tiles[ti];
     const BatchJob &j = jobs[tile.job];
     FsArgs a = fs_args_of(j);
-    switch (tile.op) {
-    case BT_FS_A: // fastconv pass 1: x -> y, column FFT * twiddle
+    if constexpr (PASS == 0) { // column FFT * twiddle: fastconv x -> y, Fftm x -> workspace
+      const bool fft = tile.op == BT_FS_FFT_A;
+      if (fft) a.scale = j.scale;
       a.in = j.x;
       a.ld_in = j.ldx;
-      a.out = j.y;
-      a.ld_out = j.ldy;
+      a.out = fft ? j.ws : j.y;
+      a.ld_out = fft ? (long)j.n : j.ldy;
       fs_col_body<L, F::P, F::C, 0>(a, tile.a, tile.b, tid, lds);
-      break;
-    case BT_FS_B: // fastconv pass 2 in place on y
-      a.in = j.y;
-      a.ld_in = j.ldy;
-      a.out = j.y;
-      a.ld_out = j.ldy;
-      fs_row_body<L, F::P, F::C, 0>(a, tile.a, tile.b, tid, lds);
-      break;
-    case BT_FS_C: // fastconv pass 3 in place on y: column IFFT
+    } else if constexpr (PASS == 1) {
+      if (tile.op == BT_FS_B) { // fastconv: row FFT * W * IFFT * twiddle in place on y
+        a.in = j.y;
+        a.ld_in = j.ldy;
+        a.out = j.y;
+        a.ld_out = j.ldy;
+        fs_row_body<L, F::P, F::C, 0>(a, tile.a, tile.b, tid, lds);
+      } else { // Fftm: workspace -> y in natural order, scaled
+        a.scale = j.scale;
+        a.in = j.ws;
+        a.ld_in = j.n;
+        a.out = j.y;
+        a.ld_out = j.ldy;
+        fs_row_body<L, F::P, F::C, 1>(a, tile.a, tile.b, tid, lds);
+      }
+    } else { // fastconv pass 3: column IFFT in place on y
       a.in = j.y;
       a.ld_in = j.ldy;
       a.out = j.y;
       a.ld_out = j.ldy;
       fs_col_body<L, F::P, F::C, 1>(a, tile.a, tile.b, tid, lds);
-      break;
-    case BT_FS_FFT_A: // forward Fftm pass 1: x -> workspace
-      a.scale = j.scale;
-      a.in = j.x;
-      a.ld_in = j.ldx;
-      a.out = j.ws;
-      a.ld_out = j.n;
-      fs_col_body<L, F::P, F::C, 0>(a, tile.a, tile.b, tid, lds);
-      break;
-    case BT_FS_FFT_B: // pass 2: workspace -> y in natural order, scaled
-      a.scale = j.scale;
-      a.in = j.ws;
-      a.ld_in = j.n;
-      a.out = j.y;
-      a.ld_out = j.ldy;
-      fs_row_body<L, F::P, F::C, 1>(a, tile.a, tile.b, tid, lds);
-      break;
-    default: break;
     }
     __syncthreads(); // LDS reuse by the next tile
   }
 }
 
-hipError_t launch_batch_fs(int n_line, const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid,
-                           hipStream_t s)
+template <int L>
+static hipError_t launch_fs_pass(int pass, const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid,
+                                 hipStream_t s)
 {
-  if (n_line == 256)
-    hipLaunchKernelGGL(k_batch_fs<256>, dim3(grid), dim3(FsCfg<256>::NT), 0, s, jobs, tiles, ntiles);
-  else if (n_line == 512)
-    hipLaunchKernelGGL(k_batch_fs<512>, dim3(grid), dim3(FsCfg<512>::NT), 0, s, jobs, tiles, ntiles);
-  else
-    return hipErrorInvalidValue;
+  constexpr int NT = FsCfg<L>::NT;
+  if (pass == 0) hipLaunchKernelGGL((k_batch_fs<L, 0>), dim3(grid), dim3(NT), 0, s, jobs, tiles, ntiles);
+  else if (pass == 1) hipLaunchKernelGGL((k_batch_fs<L, 1>), dim3(grid), dim3(NT), 0, s, jobs, tiles, ntiles);
+  else hipLaunchKernelGGL((k_batch_fs<L, 2>), dim3(grid), dim3(NT), 0, s, jobs, tiles, ntiles);
   return hipGetLastError();
+}
+
+hipError_t launch_batch_fs(int n_line, int pass, const BatchJob *jobs, const BatchTile *tiles, const int *ntiles,
+                           int grid, hipStream_t s)
+{
+  if (grid <= 0) return hipSuccess;
+  if (n_line == 256) return launch_fs_pass<256>(pass, jobs, tiles, ntiles, grid, s);
+  if (n_line == 512) return launch_fs_pass<512>(pass, jobs, tiles, ntiles, grid, s);
+  return hipErrorInvalidValue;
 }
 
 // Workgroups of a four-step pass that fit one CU together (LDS-bound).

@@ -72,10 +72,11 @@ bool batch_supported(int path, int n);
 // tiles with an atomic counter (which this launch resets first).
 hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int *counter, int grid,
                         hipStream_t s);
-// One four-step pass over tiles[0 .. *ntiles) of line length n_line (256 or
-// 512): `grid` workgroups of one line group each take tiles
+// Four-step pass `pass` (0, 1, 2) over tiles[0 .. *ntiles) of line length
+// n_line (256 or 512): `grid` workgroups of one line group each take tiles
 // blockIdx.x, blockIdx.x + grid, ... (no shared counter).
-hipError_t launch_batch_fs(int n_line, const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid,
-                           hipStream_t s);
+hipError_t launch_batch_fs(int n_line, int pass, const BatchJob *jobs, const BatchTile *tiles, const int *ntiles,
+                           int grid, hipStream_t s);
+inline int fs_queue_pass(int q) { return (q - 1) / 2; }
 
 } // namespace ovx

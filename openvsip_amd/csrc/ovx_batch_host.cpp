@@ -152,7 +152,7 @@ hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[k
     // standalone passes do); the server's graph, whose counts change per
     // batch, strides a resident grid over the queue
     const int grid = all ? batch_fs_blocks_per_cu(line) * device_cu_count() : counts[k];
-    e = launch_batch_fs(line, jobs, tiles[k], meta + k, grid, cap);
+    e = launch_batch_fs(line, fs_queue_pass(k), jobs, tiles[k], meta + k, grid, cap);
   }
   return e;
 }
@@ -167,32 +167,33 @@ void GraphBatch::release_graph()
   graph = nullptr;
 }
 
-// Upload jobs and the tile queues; capture the launches once.
-int GraphBatch::build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> q[kBatchQueues], hipStream_t s)
+// Upload jobs and the tile queues; capture the launches once, in queue order.
+int GraphBatch::build(const std::vector<BatchJob> &jobs, const std::vector<TileQueue> &queues, hipStream_t s)
 {
   release_graph();
+  const int nq = (int)queues.size();
   size_t nt = 0;
-  for (int k = 0; k < kBatchQueues; ++k) nt += q[k].size();
+  for (const auto &q : queues) nt += q.tiles.size();
   int st;
   if ((st = d_jobs.alloc(std::max<size_t>(jobs.size(), 1) * sizeof(BatchJob))) ||
-      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc(kMetaInts * sizeof(int))))
+      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc((nq + 1) * sizeof(int))))
     return st;
   std::vector<BatchTile> all;
   all.reserve(nt);
-  int counts[kMetaInts] = { 0 };
-  BatchTile *tiles[kBatchQueues];
+  std::vector<int> counts(nq + 1, 0); // tile counts, then the persistent launch's counter
+  std::vector<size_t> offset(nq);
   nlaunch = 0;
-  for (int k = 0; k < kBatchQueues; ++k) {
-    tiles[k] = (BatchTile *)d_tiles.p + all.size();
-    counts[k] = (int)q[k].size();
-    all.insert(all.end(), q[k].begin(), q[k].end());
+  for (int k = 0; k < nq; ++k) {
+    offset[k] = all.size();
+    counts[k] = (int)queues[k].tiles.size();
+    all.insert(all.end(), queues[k].tiles.begin(), queues[k].tiles.end());
     if (counts[k]) ++nlaunch;
   }
   hipError_t e = hipSuccess;
   if (!jobs.empty()) e = hipMemcpyAsync(d_jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s);
   if (e == hipSuccess && !all.empty())
     e = hipMemcpyAsync(d_tiles.p, all.data(), all.size() * sizeof(BatchTile), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_meta.p, counts, sizeof(counts), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_meta.p, counts.data(), counts.size() * sizeof(int), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_fail(e, "batch table upload");
   ntiles = nt;
@@ -203,7 +204,14 @@ int GraphBatch::build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> 
   if (e != hipSuccess) return hip_fail(e, "capture stream");
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    e = capture_batch_launches((const BatchJob *)d_jobs.p, tiles, (int *)d_meta.p, counts, false, cap);
+    const BatchJob *dj = (const BatchJob *)d_jobs.p;
+    int *meta = (int *)d_meta.p;
+    for (int k = 0; k < nq && e == hipSuccess; ++k) {
+      if (!counts[k]) continue;
+      const BatchTile *t = (const BatchTile *)d_tiles.p + offset[k];
+      if (queues[k].line == 0) e = launch_batch(dj, t, meta + k, meta + nq, device_cu_count(), cap);
+      else e = launch_batch_fs(queues[k].line, queues[k].pass, dj, t, meta + k, counts[k], cap); // one workgroup per tile
+    }
     hipError_t e2 = hipStreamEndCapture(cap, &graph);
     if (e == hipSuccess) e = e2;
   }
@@ -278,7 +286,6 @@ extern "C" int ovx_batch_commit(ovx_batch b, void *stream)
   if (!b) return fail(OVX_ERR_INVALID, "null batch");
   if (b->committed) return fail(OVX_ERR_STATE, "batch already committed");
   hipStream_t s = (hipStream_t)stream;
-  std::vector<BatchTile> q[kBatchQueues];
   b->djobs.clear();
   b->paths.clear();
   b->preps.clear();
@@ -301,8 +308,17 @@ extern "C" int ovx_batch_commit(ovx_batch b, void *stream)
   std::vector<int> order(b->djobs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->djobs[a].n > b->djobs[c].n; });
+  // Launch order: the persistent launch over every LDS-resident tile, then
+  // one launch per non-empty four-step (pass, line length) queue in pass
+  // order.  (Cutting the four-step jobs into chunks that fit the Infinity
+  // Cache, each chunk's passes back to back, measured no faster.)
+  std::vector<BatchTile> q[kBatchQueues];
   for (int i : order) make_batch_tiles(b->djobs[i], i, q);
-  int st = b->gb.build(b->djobs, q, s);
+  std::vector<TileQueue> queues;
+  for (int k = 0; k < kBatchQueues; ++k)
+    if (!q[k].empty())
+      queues.push_back(TileQueue{ k == 0 ? 0 : fs_queue_line(k), k == 0 ? 0 : fs_queue_pass(k), std::move(q[k]) });
+  int st = b->gb.build(b->djobs, queues, s);
   if (st) return st;
   b->committed = true;
   return OVX_OK;

@@ -28,11 +28,21 @@ hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[k
 // Prepared spectrum of a resident replica in the layout of path `f`.
 int replica_prepared(int id, const float2 **prep, long *n, const LineFFT *f);
 
-// Device job/tile tables plus the batch's launches captured in one hipGraph.
-// meta: tile counts [0 .. kBatchQueues), persistent-launch counter at
-// kMetaCounter.
+// Server slot tables: tile counts [0 .. kBatchQueues), persistent-launch
+// counter at kMetaCounter.
 constexpr int kMetaInts = 16;
 constexpr int kMetaCounter = 8;
+
+// One launch of a batch: the persistent kernel over LDS-resident tiles
+// (line == 0) or four-step pass `pass` over lines of length `line`.
+struct TileQueue {
+  int line = 0;
+  int pass = 0;
+  std::vector<BatchTile> tiles;
+};
+
+// Device job/tile tables plus the batch's launches (queues in launch order)
+// captured in one hipGraph.
 struct GraphBatch {
   DevBuf d_jobs, d_tiles, d_meta;
   hipGraph_t graph = nullptr;
@@ -41,7 +51,7 @@ struct GraphBatch {
   size_t ntiles = 0;
   ~GraphBatch();
   void release_graph();
-  int build(const std::vector<BatchJob> &jobs, std::vector<BatchTile> q[kBatchQueues], hipStream_t s);
+  int build(const std::vector<BatchJob> &jobs, const std::vector<TileQueue> &queues, hipStream_t s);
   int launch(hipStream_t s) const;
 };
 

@@ -104,3 +104,28 @@ def test_batch_four_step_only_and_mixed_line_lengths(ovx):
         assert o.rel_l2(y, o.fastconv64(x, reps[n][1])) < 1e-5, n
         alone = ovx.Fastconv(rows, n).with_replica(xd, reps[n][0])
         assert np.array_equal(host(alone), y), n
+
+
+def test_batch_large_four_step_jobs_bitwise_equal_standalone(ovx):
+    """Two 256-row jobs of 65536 (128 MiB each) and a small LDS-resident job:
+    one persistent launch plus one launch per four-step pass; results are
+    bitwise those of the standalone kernels."""
+    import torch
+    rng = np.random.default_rng(42)
+    jobs = []
+    reps = {}
+    for n, rows in [(65536, 256), (65536, 256), (1024, 8)]:
+        if n not in reps:
+            reps[n] = ovx.Replica(n, wl.cgauss(rng, n), is_spectrum=True)
+        xd = torch.randn(rows, n, dtype=torch.complex64, device="cuda")
+        yd = torch.full_like(xd, float("nan"))
+        jobs.append((n, rows, xd, yd))
+    b = ovx.Batch()
+    for n, rows, xd, yd in jobs:
+        b.add(xd, reps[n], yd)
+    b.commit()
+    assert b.launches == 1 + 3
+    b()
+    for n, rows, xd, yd in jobs:
+        alone = ovx.Fastconv(rows, n, variant=0).with_replica(xd, reps[n])
+        assert np.array_equal(host(alone), host(yd)), (n, rows)
