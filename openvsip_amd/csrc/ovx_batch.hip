@@ -27,21 +27,39 @@ constexpr int kBatchLds = 4 * 16 * 273 > 17408 ? 4 * 16 * 273 : 17408;
 // by the persistent loop); the tile functions read it from there.
 typedef __attribute__((address_space(3))) const BatchJob lds_job;
 
-template <int N>
-__device__ __noinline__ void tile_hier_fastconv(lds_job &j, int row0, lds_ptr lds)
+// The tile functions are called (not inlined), so the compiler cannot see
+// that the descriptor is the same for every lane: pointers read from LDS
+// would sit in vector registers and every buffer access built on them would
+// need a loop over the distinct values.  Each field a tile function uses is
+// read into scalar registers through uniform().
+template <typename T>
+__device__ __forceinline__ T uniform(const __attribute__((address_space(3))) T &f)
 {
+  static_assert(sizeof(T) % 4 == 0, "dword copy");
+  T u;
+  const __attribute__((address_space(3))) int *src = (const __attribute__((address_space(3))) int *)&f;
+  int *dst = (int *)&u;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return u;
+}
+
+template <int N>
+__device__ __noinline__ void tile_hier_fastconv(lds_job &j, int row0_arg, lds_ptr lds)
+{
+  const int row0 = __builtin_amdgcn_readfirstlane(row0_arg); // uniform, see uniform()
   using H = Hier<N>;
   static_assert((kBatchThreads / H::TR) * H::LROW <= kBatchLds, "tile LDS");
   HierArgs a{};
-  a.x = j.x;
-  a.ldx = j.ldx;
-  a.y = j.y;
-  a.ldy = j.ldy;
-  a.w = j.w;
+  a.x = uniform(j.x);
+  a.ldx = uniform(j.ldx);
+  a.y = uniform(j.y);
+  a.ldy = uniform(j.ldy);
+  a.w = uniform(j.w);
   a.ldw = 0;
-  a.rows = j.rows;
-  a.tw_outer = j.tw0;
-  a.tw_wave = j.tw1;
+  a.rows = uniform(j.rows);
+  a.tw_outer = uniform(j.tw0);
+  a.tw_wave = uniform(j.tw1);
   a.scale = 1.0f;
   a.natural = 0;
   const int lrow = (int)threadIdx.x / H::TR;
@@ -50,67 +68,57 @@ __device__ __noinline__ void tile_hier_fastconv(lds_job &j, int row0, lds_ptr ld
 }
 
 template <int N>
-__device__ __noinline__ void tile_hier_fwd(lds_job &j, int row0, lds_ptr lds)
+__device__ __noinline__ void tile_hier_fwd(lds_job &j, int row0_arg, lds_ptr lds)
 {
+  const int row0 = __builtin_amdgcn_readfirstlane(row0_arg); // uniform, see uniform()
   using H = Hier<N>;
   static_assert((kBatchThreads / H::TR) * H::LROW <= kBatchLds, "tile LDS");
   HierArgs a{};
-  a.x = j.x;
-  a.ldx = j.ldx;
-  a.y = j.y;
-  a.ldy = j.ldy;
-  a.rows = j.rows;
-  a.tw_outer = j.tw0;
-  a.tw_wave = j.tw1;
-  a.scale = j.scale;
+  a.x = uniform(j.x);
+  a.ldx = uniform(j.ldx);
+  a.y = uniform(j.y);
+  a.ldy = uniform(j.ldy);
+  a.rows = uniform(j.rows);
+  a.tw_outer = uniform(j.tw0);
+  a.tw_wave = uniform(j.tw1);
+  a.scale = uniform(j.scale);
   const int lrow = (int)threadIdx.x / H::TR;
   const int t = (int)threadIdx.x % H::TR;
   hier_fwd_body<N>(a, row0, lrow, t, lds + lrow * H::LROW);
 }
 
 template <int N, int P>
-__device__ __noinline__ void tile_row_fastconv(lds_job &j, int row0, lds_ptr lds)
+__device__ __noinline__ void tile_row_fastconv(lds_job &j, int row0_arg, lds_ptr lds)
 {
+  const int row0 = __builtin_amdgcn_readfirstlane(row0_arg); // uniform, see uniform()
   using E = Engine<N, P, (N / P <= 64)>;
   static_assert((kBatchThreads / E::T) * E::LROW <= kBatchLds, "tile LDS");
   const int lrow = (int)threadIdx.x / E::T;
   const int t = (int)threadIdx.x % E::T;
-  row_fastconv_body<N, P>(j.x, j.ldx, j.w, 0, j.y, j.ldy, j.rows, j.tw0, 1.0f, row0, lrow, t,
-                          lds + lrow * E::LROW);
+  row_fastconv_body<N, P>(uniform(j.x), uniform(j.ldx), uniform(j.w), 0, uniform(j.y), uniform(j.ldy),
+                          uniform(j.rows), uniform(j.tw0), 1.0f, row0, lrow, t, lds + lrow * E::LROW);
 }
 
 template <int N, int P>
-__device__ __noinline__ void tile_row_fwd(lds_job &j, int row0, lds_ptr lds)
+__device__ __noinline__ void tile_row_fwd(lds_job &j, int row0_arg, lds_ptr lds)
 {
+  const int row0 = __builtin_amdgcn_readfirstlane(row0_arg); // uniform, see uniform()
   using E = Engine<N, P, (N / P <= 64)>;
   static_assert((kBatchThreads / E::T) * E::LROW <= kBatchLds, "tile LDS");
   const int lrow = (int)threadIdx.x / E::T;
   const int t = (int)threadIdx.x % E::T;
-  row_fft_body<N, P, -1>(j.x, j.ldx, j.y, j.ldy, j.rows, j.tw0, j.scale, row0, lrow, t, lds + lrow * E::LROW);
+  row_fft_body<N, P, -1>(uniform(j.x), uniform(j.ldx), uniform(j.y), uniform(j.ldy), uniform(j.rows),
+                         uniform(j.tw0), uniform(j.scale), row0, lrow, t, lds + lrow * E::LROW);
 }
 
 // Overlap-save segments seg0, seg0 + 1, ... (kBatchThreads / (L/16) of them
 // per tile); a copy past the last segment runs with P = 0, which stores
 // nothing, because ols_body's barriers need every thread.
 template <int L>
-__device__ __noinline__ void tile_ols(lds_job &j, int seg0, lds_ptr lds)
+__device__ __noinline__ void tile_ols(lds_job &j, int seg0_arg, lds_ptr lds)
 {
-  OlsArgs a;
-  a.in = j.ols.in;
-  a.N = j.ols.N;
-  a.M = j.ols.M;
-  a.H = j.ols.H;
-  a.tw = j.ols.tw;
-  a.scale = j.ols.scale;
-  a.conj_input = j.ols.conj_input;
-  a.unbiased = j.ols.unbiased;
-  a.support = j.ols.support;
-  a.decimation = j.ols.decimation;
-  a.out_off = j.ols.out_off;
-  a.c_first = j.ols.c_first;
-  a.c_step = j.ols.c_step;
-  a.P = j.ols.P;
-  a.out = j.ols.out;
+  const int seg0 = __builtin_amdgcn_readfirstlane(seg0_arg); // uniform, see uniform()
+  OlsArgs a = uniform(j.ols);
   constexpr int T = L / 16;
   static_assert(T <= kBatchThreads, "segment too long");
   static_assert((kBatchThreads / T) * Engine<L, 16>::LROW <= kBatchLds, "tile LDS");
