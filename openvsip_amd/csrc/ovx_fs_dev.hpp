@@ -83,7 +83,9 @@ __device__ __forceinline__ void fs_col_body(const FsArgs &a, long row, long grou
 template <int L, int P, int C, int MODE, typename LP>
 __device__ __forceinline__ void fs_row_body(const FsArgs &a, long row, long group, int tid, LP lds_all)
 {
-  using E = Engine<L, P>;
+  // a line's L/P threads are consecutive; when they fit one wavefront its
+  // exchanges need no workgroup barrier
+  using E = Engine<L, P, (L / P <= 64)>;
   constexpr int T = E::T;
   constexpr int LROW = FsLds<L>::LROW;
   const int c = tid / T; // consecutive threads walk along a line
