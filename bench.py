@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--variant", type=int, default=None, help="force a compiled kernel variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-jobs measurement")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-lengths", default="",
+                    help="comma-separated N: keep only the batch's jobs of these lengths (per-length breakdown)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16x3 MFMA variant measurement")
     ap.add_argument("--copy-probe", action="store_true",
@@ -161,15 +163,25 @@ def batch_section(args):
     from openvsip_amd import workloads as wl
 
     spec = wl.batch_jobs(seed=3)
+    keep = {int(v) for v in args.batch_lengths.split(",") if v}
     gen = torch.Generator(device="cuda")
     gen.manual_seed(3)
     reps, jobs = {}, []
+    if keep:
+        # the inputs of the kept jobs are drawn exactly as in the full batch
+        kept = []
+        for n, rows in spec:
+            x = torch.randn(rows, n, dtype=torch.complex64, device="cuda", generator=gen)
+            if n in keep:
+                kept.append((n, rows, x))
+        spec = [(n, rows) for n, rows, _ in kept]
+        drawn = iter([x for _, _, x in kept])
     for n in sorted({n for n, _ in spec}):
         reps[n] = ovx.Replica(n, wl.lfm_chirp(n, n // 8).astype(np.complex64))
     b = ovx.Batch()
     alg = 0
     for n, rows in spec:
-        x = torch.randn(rows, n, dtype=torch.complex64, device="cuda", generator=gen)
+        x = next(drawn) if keep else torch.randn(rows, n, dtype=torch.complex64, device="cuda", generator=gen)
         y = torch.empty_like(x)
         b.add(x, reps[n], y)
         jobs.append((n, rows, x, y))
@@ -203,7 +215,8 @@ def batch_section(args):
     total_rows = sum(r for _, r in spec)
     return {
         "workload": "512 fastconv jobs, seed 3, N in {512,1024,2048,4096,8192,16384,65536,3072,6144,12288}, "
-                    "rows U[16,256]; replica LFM chirp (BT = N/8) resident per N",
+                    "rows U[16,256]; replica LFM chirp (BT = N/8) resident per N"
+                    + (f"; only the jobs of N in {sorted(keep)}" if keep else ""),
         "jobs": len(spec), "rows": total_rows, "tiles": b.tiles, "launches_per_replay": b.launches,
         "batched_s": t_batch, "plain_s": t_plain,
         "batched_rows_per_s": total_rows / t_batch, "plain_rows_per_s": total_rows / t_plain,
@@ -369,6 +382,22 @@ def main():
     _lib.check(fn(*args_c))
     torch.cuda.synchronize()
 
+    # practical ceiling for the same bytes: a device copy X -> Y timed the
+    # same way (K launches between two events on the same stream)
+    for _ in range(args.warmup):
+        yd.copy_(xd)
+    torch.cuda.synchronize()
+    c0 = torch.cuda.Event(enable_timing=True)
+    c1 = torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    for _ in range(args.steps):
+        yd.copy_(xd)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    copy_ms = c0.elapsed_time(c1) / args.steps
+    _lib.check(fn(*args_c))  # restore the benchmarked output for the parity check
+    torch.cuda.synchronize()
+
     # parity of the benchmarked output on a subset of rows
     import oracle
     idx = list(range(0, R, max(1, R // 8)))[:8]
@@ -399,6 +428,10 @@ def main():
                    "autotuned_variant": None if tuned is None else tuned[0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(fc.name, R, N)},
+        "hbm_copy": {"what": "device copy of X into Y (2*R*N*8 bytes), same timing: the practical "
+                             "ceiling of a pass that reads X once and writes Y once",
+                     "ms": copy_ms, "GBs": 2 * R * N * 8 / (copy_ms * 1e-3) / 1e9,
+                     "kernel_vs_copy": copy_ms / ms},
         "gflops": flops / sec / 1e9,
         "parity": {"rel_l2_vs_fp64": rel, "rows_checked": len(idx)},
     }
