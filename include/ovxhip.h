@@ -94,7 +94,8 @@ const char *ovx_last_error(void);
 const char *ovx_version(void);
 /* Largest transform length the backend accepts (rt_valid upper bound). */
 long ovx_max_length(void);
-/* 1 if `n` is a transform length this backend handles, else 0. */
+/* 1 if `n` is a transform length this backend handles, else 0: 2^k, 3*2^k,
+   5*2^k and 9*2^k up to 16384, and 2^k from 32768 to ovx_max_length(). */
 int ovx_supported_length(long n);
 /* Number of compiled kernel variants for transform length n. */
 int ovx_length_variants(long n);

@@ -173,7 +173,9 @@ static hipError_t do_fastconv(const float2 *x, long ldx, const float2 *w, long l
   X(384, 24) X(768, 24) X(1536, 24) X(3072, 24) X(6144, 24) X(12288, 24)       \
   X(3072, 12) X(6144, 12) X(12288, 12)                                          \
   X(5, 5) X(10, 10) X(20, 20) X(40, 40) X(80, 40) X(160, 40) X(320, 40)         \
-  X(640, 40) X(1280, 40) X(2560, 40) X(5120, 40) X(10240, 40)
+  X(640, 40) X(1280, 40) X(2560, 40) X(5120, 40) X(10240, 40)                 \
+  X(9, 9) X(18, 18) X(36, 36) X(72, 36) X(144, 36) X(288, 36) X(576, 36)       \
+  X(1152, 36) X(2304, 36) X(4608, 36) X(9216, 36)
 
 struct Entry {
   int n, p;

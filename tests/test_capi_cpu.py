@@ -41,8 +41,10 @@ def test_output_sizes_match_oracle():
 
 def test_supported_lengths():
     lib = _lib.lib()
-    for n in (2, 16, 2048, 3072, 12288, 16384, 10240):
+    for n in (2, 16, 2048, 3072, 12288, 16384, 10240, 9, 18, 9216, 65536, 262144):
         assert lib.ovx_supported_length(n) == 1, n
+    for n in (7, 19, 68, 242, 252, 27, 16385):
+        assert lib.ovx_supported_length(n) == 0, n
     assert lib.ovx_max_length() >= 16384
 
 

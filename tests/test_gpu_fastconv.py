@@ -35,7 +35,8 @@ def say(*a):
 
 
 POW2 = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
-SMOOTH = [3, 6, 12, 24, 48, 96, 768, 1536, 3072, 6144, 12288, 5, 10, 20, 40, 80, 640, 5120, 10240]
+SMOOTH = [3, 6, 12, 24, 48, 96, 768, 1536, 3072, 6144, 12288, 5, 10, 20, 40, 80, 640, 5120, 10240,
+          9, 18, 36, 72, 144, 576, 2304, 9216]
 
 
 @pytest.mark.parametrize("n", POW2 + SMOOTH)
@@ -231,7 +232,7 @@ def test_fastconv_coefficient_matrix_and_in_place(ovx):
     assert o.rel_l2(y, ref) < REL_L2
 
 
-@pytest.mark.parametrize("n", [512, 1024, 3072, 6144, 8192, 12288, 10240])
+@pytest.mark.parametrize("n", [512, 1024, 3072, 6144, 8192, 12288, 10240, 18, 4608])
 def test_fastconv_sizes(ovx, n):
     rows = 19
     rng = np.random.default_rng(n + 1)

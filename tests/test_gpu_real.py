@@ -25,7 +25,8 @@ def say(*a):
 
 # n: real length; the complex half-length h = n/2 covers the register-Stockham
 # (32, 6144), hierarchical (2048, 32768) and four-step (131072) paths
-@pytest.mark.parametrize("n,rows", [(4, 3), (32, 5), (1024, 33), (2048, 16), (6144, 9), (32768, 12), (131072, 4)])
+@pytest.mark.parametrize("n,rows", [(4, 3), (18, 7), (32, 5), (72, 3), (1024, 33), (2048, 16), (6144, 9),
+                                    (32768, 12), (131072, 4)])
 def test_r2c_rows_vs_fp64(ovx, n, rows):
     import torch
     rng = np.random.default_rng(n + rows)
@@ -39,7 +40,8 @@ def test_r2c_rows_vs_fp64(ovx, n, rows):
     assert err < REL_L2
 
 
-@pytest.mark.parametrize("n,rows", [(4, 3), (32, 5), (1024, 33), (2048, 16), (6144, 9), (32768, 12), (131072, 4)])
+@pytest.mark.parametrize("n,rows", [(4, 3), (18, 7), (32, 5), (72, 3), (1024, 33), (2048, 16), (6144, 9),
+                                    (32768, 12), (131072, 4)])
 def test_c2r_rows_vs_fp64(ovx, n, rows):
     import torch
     rng = np.random.default_rng(2 * n + rows)
