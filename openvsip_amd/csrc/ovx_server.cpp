@@ -120,6 +120,7 @@ struct Server {
   unsigned long next_id = 1;
   size_t budget = 0, in_use = 0;
   bool stop = false;
+  bool sched_done = false;       // the scheduler has exited: nothing more enters `inflight`
   unsigned long completions = 0; // bumped by every retired slot
   int error = OVX_OK;
   std::thread sched, compl_;
@@ -147,8 +148,13 @@ Server::~Server()
     stop = true;
   }
   cv_sched.notify_all();
-  cv_done.notify_all();
   if (sched.joinable()) sched.join();
+  // the completer retires every slot the scheduler launched before it exits
+  {
+    std::lock_guard<std::mutex> g(mu);
+    sched_done = true;
+  }
+  cv_done.notify_all();
   if (compl_.joinable()) compl_.join();
   for (auto &s : slots) {
     if (s->exec) (void)hipGraphExecDestroy(s->exec);
@@ -289,7 +295,7 @@ int Server::admit_and_launch(Slot &s, std::vector<Req *> &batch)
     std::copy(q[k].begin(), q[k].end(), s.h_tiles[k]);
     meta[k] = (int)q[k].size();
   }
-  std::copy(meta, meta + 8, s.h_meta);
+  std::copy(meta, meta + kMetaInts, s.h_meta);
   hipError_t e = hipMemcpyAsync(s.d_jobs.p, s.h_jobs, jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s.stream);
   for (int k = 0; k < kBatchQueues && e == hipSuccess; ++k)
     if (!q[k].empty())
@@ -413,8 +419,8 @@ void Server::completer()
     Slot *s = nullptr;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_done.wait(lk, [&] { return stop || !inflight.empty(); });
-      if (inflight.empty()) return; // stop and nothing in flight
+      cv_done.wait(lk, [&] { return sched_done || !inflight.empty(); });
+      if (inflight.empty()) return; // the scheduler has exited and nothing is in flight
       s = inflight.front();
     }
     hipError_t e = hipEventSynchronize(s->done);

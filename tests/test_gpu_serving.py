@@ -99,3 +99,24 @@ def test_server_refuses_over_budget(ovx):
         srv.wait(t)
     assert ei.value.code == 5  # OVX_ERR_BUDGET
     srv.close()
+
+
+def test_server_close_with_requests_in_flight(ovx):
+    """Closing a server whose slots still run batches (and whose queue still
+    holds requests) returns once the launched batches have retired; the
+    device stays usable and every output a launched batch wrote is final."""
+    import time
+    import torch
+    srv = ovx.Server(slots=3, max_jobs=2, hbm_budget_mb=4096, workspace_mb=64)
+    w = wl.replica_spectrum(wl.lfm_chirp(16384, 2048), 16384)
+    rep = ovx.Replica(16384, w, is_spectrum=True)
+    xs = [torch.randn((64, 16384), dtype=torch.complex64, device="cuda") for _ in range(4)]
+    ys = [torch.empty_like(x) for x in xs]
+    for i in range(40):
+        srv.submit(ovx.fastconv_request(xs[i % 4], rep, ys[i % 4]))
+    t0 = time.time()
+    srv.close()
+    assert time.time() - t0 < 30.0
+    torch.cuda.synchronize()
+    z = torch.ones(4, device="cuda") * 2
+    assert float(z.sum()) == 8.0
