@@ -203,7 +203,9 @@ int ovx_fastconv_exec_replica(ovx_plan plan, const void *x, long ldx, int replic
    per workgroup).  The launches are captured once into a hipGraph at commit
    and replayed by every exec.  Each job's result is bitwise the same as
    ovx_fastconv_exec_replica on the same job.  ovx_batch_launches returns
-   the number of kernel launches per replay. */
+   the number of kernel launches per replay.  A committed batch holds the
+   device addresses of its jobs' buffers and replica spectra: they must stay
+   valid (and the replicas not destroyed) until the batch is destroyed. */
 typedef struct ovx_batch_s *ovx_batch;
 int ovx_batch_create(ovx_batch *batch);
 int ovx_batch_add_fastconv(ovx_batch batch, long n, long rows, const void *x,
@@ -224,8 +226,11 @@ int ovx_batch_destroy(ovx_batch batch);
    tile-queue launches captured at creation -- and replays it; requests that
    arrive meanwhile go into the next free slot, so batches overlap on the
    device.  Data pointers are device pointers owned by the caller and must
-   stay valid until the request completes.  plain != 0 runs every request as
-   its own standalone launch instead (the per-request path). */
+   stay valid until the request completes, and the replicas and correlation
+   references a request names until it completes.  plain != 0 runs every
+   request as its own standalone launch instead (the per-request path).
+   ovx_server_destroy waits for the batches already launched; requests still
+   queued are dropped. */
 typedef struct ovx_server_s *ovx_server;
 typedef struct {
   int slots;          /* batches in flight at once (>= 1)                    */

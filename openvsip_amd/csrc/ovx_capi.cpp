@@ -628,6 +628,7 @@ extern "C" int ovx_fftm_real_out_of_place(ovx_plan p, const void *in, long in_s0
   if ((st = real_side(p, fwd ? out_s0 : in_s0, fwd ? out_s1 : in_s1, h + 1, &lines, &ldc))) return st;
   if (n == 0 || lines == 0) return OVX_OK; // empty views may carry null pointers
   if (!in || !out) return fail(OVX_ERR_INVALID, "null data pointer");
+  if (in == out) return fail(OVX_ERR_UNSUPPORTED, "real Fftm runs out of place (input and output differ in type)");
   if (lines > 1 && (ldr % 2))
     return fail(OVX_ERR_UNSUPPORTED, "real line stride %ld must be even (lines are read as complex pairs)", ldr);
   const void *real_ptr = fwd ? in : out;

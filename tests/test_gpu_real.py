@@ -108,3 +108,19 @@ def test_c2r_ignores_imaginary_dc_and_nyquist(ovx):
     b = host(f(torch.from_numpy(clean).cuda()))
     assert np.array_equal(a, b)
     assert o.rel_l2(a, n * np.fft.irfft(clean, n, axis=1)) < REL_L2
+
+
+def test_real_fftm_refuses_aliased_input_and_output(ovx):
+    """Input and output of a real Fftm differ in type and line length, so
+    the transform runs out of place only; the same buffer on both sides is
+    refused, not computed with overlapping reads and writes."""
+    import ctypes
+    import torch
+    from openvsip_amd import _lib
+    rows, n = 4, 64
+    buf = torch.zeros(rows * (n + 2), dtype=torch.float32, device="cuda")
+    f = ovx.FftmReal(rows, n, ovx.ROW, ovx.FFT_FWD, 1.0)
+    p = ctypes.c_void_p(buf.data_ptr())
+    st = _lib.lib().ovx_fftm_real_out_of_place(f._h, p, n + 2, 1, p, (n + 2) // 2, 1, rows, n,
+                                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert st == _lib.OVX_ERR_UNSUPPORTED
