@@ -29,7 +29,9 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2000,
+                    help="untimed steps before the timed ones; the GPU clocks take ~1500 steps (~35 ms) "
+                         "of this kernel to settle, and fewer warm-up steps time the ramp")
     ap.add_argument("--rows", type=int, default=256)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--cpu-seconds", type=float, default=10.0,

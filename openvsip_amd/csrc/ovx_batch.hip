@@ -64,7 +64,8 @@ __device__ __noinline__ void tile_hier_fastconv(lds_job &j, int row0_arg, lds_pt
   a.natural = 0;
   const int lrow = (int)threadIdx.x / H::TR;
   const int t = (int)threadIdx.x % H::TR;
-  hier_fastconv_body<N>(a, row0, lrow, t, lds + lrow * H::LROW);
+  // a tile of one row is only made for a row that exists
+  hier_fastconv_body<N, kBatchThreads / H::TR == 1>(a, row0, lrow, t, lds + lrow * H::LROW);
 }
 
 template <int N>

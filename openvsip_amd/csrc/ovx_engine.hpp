@@ -74,7 +74,13 @@ template <int N, int P>
 struct LdsLayout {
   static constexpr bool kPad32 = (N / P) >= 16 && N >= 64;
   static constexpr int elems = kPad32 ? N + (N + 31) / 32 : lds_row_elems(N);
+  static constexpr int period = kPad32 ? 32 : 16; // one pad element per `period`
   __host__ __device__ static constexpr int idx(int pos) { return kPad32 ? pos + (pos >> 5) : pos + (pos >> 4); }
+  // idx(pos + c) == idx(pos) + step(c) for every pos >= 0 when c is a
+  // multiple of the period: the pad count of the sum splits exactly, so
+  // accesses at such distances share one base address and differ by a
+  // constant offset (the compiler does not find this by itself).
+  __host__ __device__ static constexpr int step(int c) { return c + c / period; }
 };
 
 // Pointer into LDS with the address space spelled out (for code that is not
@@ -147,14 +153,26 @@ struct Engine {
         for (int r = 0; r < R; ++r) v[m + r * M] = a[r];
       } else {
         const int base = (j / Ns) * Ns * R + (j % Ns);
+        if constexpr (Ns % Layout::period == 0) {
+          const int b = Layout::idx(base);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds_st(lds, Layout::idx(base + r * Ns), a[r]);
+          for (int r = 0; r < R; ++r) lds_st(lds, b + r * Layout::step(Ns), a[r]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) lds_st(lds, Layout::idx(base + r * Ns), a[r]);
+        }
       }
     }
     if constexpr (!last) {
       exchange_sync<WAVE>();
+      if constexpr (T % Layout::period == 0) {
+        const int b = Layout::idx(t);
 #pragma unroll
-      for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, Layout::idx(t + T * q));
+        for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, b + q * Layout::step(T));
+      } else {
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, Layout::idx(t + T * q));
+      }
       exchange_sync<WAVE>();
     }
   }

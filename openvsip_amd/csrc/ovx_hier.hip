@@ -28,7 +28,8 @@ __global__ __launch_bounds__(Hier<N>::NT) void k_hier_fastconv(HierArgs a)
   __shared__ float2 lds_all[H::B * H::LROW];
   const int lrow = (H::B == 1) ? 0 : (int)threadIdx.x / H::TR;
   const int t = (H::B == 1) ? (int)threadIdx.x : (int)threadIdx.x % H::TR;
-  hier_fastconv_body<N>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
+  // one row per block and a grid of exactly `rows` blocks: every row exists
+  hier_fastconv_body<N, H::B == 1>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
 }
 
 template <int N>

@@ -79,13 +79,15 @@ __device__ __forceinline__ void outer_inverse_store(const rsrc_t &ry, uint32_t v
 // rows row0 + lrow of the job, thread t of the row, LDS row region `lds`.
 // Every thread of the workgroup must call it (it contains workgroup
 // barriers); rows at or beyond a.rows compute on zeros and store nothing.
-template <int N, typename LP>
+// ONE_ROW: the caller's workgroup holds a single row that exists (one row
+// per block, one block per row), so no row needs the check.
+template <int N, bool ONE_ROW = false, typename LP>
 __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0, int lrow, int t, LP lds)
 {
   using H = Hier<N>;
   constexpr int R0 = H::R0, NW = H::NW;
   const long row = row0 + lrow;
-  const bool valid = row < a.rows;
+  const bool valid = ONE_ROW || row < a.rows;
   const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
   const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
   const rsrc_t rw = make_rsrc(a.w + row0 * a.ldw, 0xffffffffu);
@@ -101,11 +103,13 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
   // ---- wave stage: wave k1 owns slice k1
   const int k1 = t >> 6;
   const int l = t & 63;
+  const int ls = H::WE::Layout::idx(l); // slot of position l; l + 64 s is at ls + s * step(64)
+  static_assert(64 % H::WE::Layout::period == 0, "constant-offset slice access");
   LP sl = lds + k1 * H::LSL;
   float2 u[16];
   if constexpr (R0 > 1) {
 #pragma unroll
-    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, H::WE::Layout::idx(l + 64 * s));
+    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, ls + s * H::WE::Layout::step(64));
   } else {
 #pragma unroll
     for (int s = 0; s < 16; ++s)
@@ -139,7 +143,7 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
         u[s] = cmulc(u[s], bload(rto, (uint32_t)((k1 - 1) * NW + l) * 8u, (uint32_t)(64 * s) * 8u));
     }
 #pragma unroll
-    for (int s = 0; s < 16; ++s) lds_st(sl, H::WE::Layout::idx(l + 64 * s), u[s]);
+    for (int s = 0; s < 16; ++s) lds_st(sl, ls + s * H::WE::Layout::step(64), u[s]);
     __syncthreads();
     outer_inverse_store<N, LP>(ry, vy, valid, lds, t, 1.0f);
   } else {
@@ -173,11 +177,13 @@ __device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int 
   }
   const int k1 = t >> 6;
   const int l = t & 63;
+  const int ls = H::WE::Layout::idx(l); // slot of position l; l + 64 s is at ls + s * step(64)
+  static_assert(64 % H::WE::Layout::period == 0, "constant-offset slice access");
   LP sl = lds + k1 * H::LSL;
   float2 u[16];
   if constexpr (R0 > 1) {
 #pragma unroll
-    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, H::WE::Layout::idx(l + 64 * s));
+    for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, ls + s * H::WE::Layout::step(64));
   } else {
 #pragma unroll
     for (int s = 0; s < 16; ++s)
@@ -222,6 +228,8 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
   const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
   const int k1 = t >> 6;
   const int l = t & 63;
+  const int ls = H::WE::Layout::idx(l); // slot of position l; l + 64 s is at ls + s * step(64)
+  static_assert(64 % H::WE::Layout::period == 0, "constant-offset slice access");
   LP sl = lds + k1 * H::LSL;
   float2 u[16];
   if constexpr (R0 > 1) {
@@ -245,7 +253,7 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
         u[s] = cmulc(u[s], bload(rto, (uint32_t)((k1 - 1) * NW + l) * 8u, (uint32_t)(64 * s) * 8u));
     }
 #pragma unroll
-    for (int s = 0; s < 16; ++s) lds_st(sl, H::WE::Layout::idx(l + 64 * s), u[s]);
+    for (int s = 0; s < 16; ++s) lds_st(sl, ls + s * H::WE::Layout::step(64), u[s]);
     __syncthreads();
     outer_inverse_store<N, LP>(ry, vy, valid, lds, t, a.scale);
   } else {
