@@ -124,6 +124,31 @@ struct Engine {
   static_assert(N % P == 0, "P must divide N");
   static_assert(plan.nst > 0 || N == 1, "no radix plan for N with this P");
 
+  // Whether the slots a thread (0 <= t < T) reads, positions t + T q, sit at
+  // idx(t) + step(T q): no pad boundary falls between them differently for
+  // different t.  Then one base address and constant offsets serve all P
+  // reads.  Checked exhaustively (N cases).
+  static constexpr bool read_split()
+  {
+    for (int t = 0; t < T; ++t)
+      for (int q = 0; q < P; ++q)
+        if (Layout::idx(t + T * q) != Layout::idx(t) + Layout::step(T * q)) return false;
+    return true;
+  }
+  // The same for the R slots stage S writes from base (j / Ns) Ns R + j % Ns,
+  // j = 0 .. N / R - 1, at distances r Ns.
+  template <int S>
+  static constexpr bool write_split()
+  {
+    constexpr int R = plan.radix[S], Ns = plan.ns[S];
+    for (int j = 0; j < N / R; ++j) {
+      const int base = (j / Ns) * Ns * R + (j % Ns);
+      for (int r = 0; r < R; ++r)
+        if (Layout::idx(base + r * Ns) != Layout::idx(base) + Layout::step(r * Ns)) return false;
+    }
+    return true;
+  }
+
   // One stage.  tw: concatenated forward twiddle table (FP32 values of FP64
   // W_{Ns R}^{r k}, layout tw[twoff_s + (r-1)*Ns + k]).
   template <int S, int DIR, typename LP>
@@ -153,10 +178,10 @@ struct Engine {
         for (int r = 0; r < R; ++r) v[m + r * M] = a[r];
       } else {
         const int base = (j / Ns) * Ns * R + (j % Ns);
-        if constexpr (Ns % Layout::period == 0) {
+        if constexpr (write_split<S>()) {
           const int b = Layout::idx(base);
 #pragma unroll
-          for (int r = 0; r < R; ++r) lds_st(lds, b + r * Layout::step(Ns), a[r]);
+          for (int r = 0; r < R; ++r) lds_st(lds, b + Layout::step(r * Ns), a[r]);
         } else {
 #pragma unroll
           for (int r = 0; r < R; ++r) lds_st(lds, Layout::idx(base + r * Ns), a[r]);
@@ -165,10 +190,10 @@ struct Engine {
     }
     if constexpr (!last) {
       exchange_sync<WAVE>();
-      if constexpr (T % Layout::period == 0) {
+      if constexpr (read_split()) {
         const int b = Layout::idx(t);
 #pragma unroll
-        for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, b + q * Layout::step(T));
+        for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, b + Layout::step(T * q));
       } else {
 #pragma unroll
         for (int q = 0; q < P; ++q) v[q] = lds_ld(lds, Layout::idx(t + T * q));
