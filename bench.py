@@ -349,18 +349,19 @@ def main():
         print(json.dumps({"copy_probe": True, "bytes_per_copy": 2 * R * N * 8, "ms_per_copy": ms,
                           "GBs": 2 * R * N * 8 / (ms * 1e-3) / 1e9}), flush=True)
         return
+    tuned = None
+    if args.variant is None:
+        # radix plan / kernel variant for (N, rows): the tuner's cached choice,
+        # tuned now (at start-up, before the data is allocated) when the
+        # cache has none
+        v = ovx.tuned_variant(N, R)
+        tuned = (v, None) if v >= 0 else ovx.autotune(N, R)
     x, w, _, _ = wl.radar(R, N, seed=1)
     xd = torch.from_numpy(x).cuda()
     yd = torch.empty_like(xd)
     # the replica spectrum is resident: transformed once, prepared once in the
     # kernel's layout (the serving path); the timed steps read X, write Y.
     rep = ovx.Replica(N, w, is_spectrum=True)
-    tuned = None
-    if args.variant is None:
-        # radix plan / kernel variant for (N, rows): the tuner's cached choice,
-        # tuned now when the cache has none
-        v = ovx.tuned_variant(N, R)
-        tuned = (v, None) if v >= 0 else ovx.autotune(N, R)
     fc = ovx.Fastconv(R, N, variant=args.variant)
     lib = _lib.lib()
     stream = torch.cuda.current_stream()
