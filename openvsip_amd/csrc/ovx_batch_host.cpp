@@ -130,6 +130,38 @@ void make_batch_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatch
   }
 }
 
+std::vector<TileQueue> chunked_fs_queues(const std::vector<BatchJob> &jobs, const std::vector<int> &order,
+                                         size_t chunk_bytes)
+{
+  std::vector<TileQueue> out;
+  std::vector<BatchTile> cq[kBatchQueues];
+  size_t used = 0;
+  auto flush = [&] {
+    for (int k = 1; k < kBatchQueues; ++k)
+      if (!cq[k].empty()) {
+        out.push_back(TileQueue{ fs_queue_line(k), fs_queue_pass(k), std::move(cq[k]) });
+        cq[k].clear();
+      }
+    used = 0;
+  };
+  for (int ji : order) {
+    const BatchJob &j = jobs[(size_t)ji];
+    if (j.path != BP_FS) continue;
+    const size_t row_bytes = (size_t)j.n * 8;
+    const int gcol = j.n2 / batch_fs_lines_per_group(j.n1), grow = j.n1 / batch_fs_lines_per_group(j.n2);
+    const int qa = fs_queue(0, j.n1), qb = fs_queue(1, j.n2), qc = fs_queue(2, j.n1);
+    for (long r = 0; r < j.rows; ++r) {
+      if (used > 0 && used + row_bytes > chunk_bytes) flush();
+      for (int g = 0; g < gcol; ++g) cq[qa].push_back(BatchTile{ ji, BT_FS_A, (int)r, g });
+      for (int g = 0; g < grow; ++g) cq[qb].push_back(BatchTile{ ji, BT_FS_B, (int)r, g });
+      for (int g = 0; g < gcol; ++g) cq[qc].push_back(BatchTile{ ji, BT_FS_C, (int)r, g });
+      used += row_bytes;
+    }
+  }
+  flush();
+  return out;
+}
+
 void make_batch_fft_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatchQueues])
 {
   const int gcol = j.n2 / batch_fs_lines_per_group(j.n1), grow = j.n1 / batch_fs_lines_per_group(j.n2);
@@ -309,15 +341,14 @@ extern "C" int ovx_batch_commit(ovx_batch b, void *stream)
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->djobs[a].n > b->djobs[c].n; });
   // Launch order: the persistent launch over every LDS-resident tile, then
-  // one launch per non-empty four-step (pass, line length) queue in pass
-  // order.  (Cutting the four-step jobs into chunks that fit the Infinity
-  // Cache, each chunk's passes back to back, measured no faster.)
+  // the four-step jobs' rows in Infinity-Cache-sized chunks, one launch per
+  // non-empty (pass, line length) queue of a chunk, in pass order.
   std::vector<BatchTile> q[kBatchQueues];
-  for (int i : order) make_batch_tiles(b->djobs[i], i, q);
+  for (int i : order)
+    if (b->djobs[i].path != BP_FS) make_batch_tiles(b->djobs[i], i, q);
   std::vector<TileQueue> queues;
-  for (int k = 0; k < kBatchQueues; ++k)
-    if (!q[k].empty())
-      queues.push_back(TileQueue{ k == 0 ? 0 : fs_queue_line(k), k == 0 ? 0 : fs_queue_pass(k), std::move(q[k]) });
+  if (!q[0].empty()) queues.push_back(TileQueue{ 0, 0, std::move(q[0]) });
+  for (auto &fq : chunked_fs_queues(b->djobs, order, kFsChunkBytes)) queues.push_back(std::move(fq));
   int st = b->gb.build(b->djobs, queues, s);
   if (st) return st;
   b->committed = true;

@@ -107,14 +107,16 @@ def test_batch_four_step_only_and_mixed_line_lengths(ovx):
 
 
 def test_batch_large_four_step_jobs_bitwise_equal_standalone(ovx):
-    """Two 256-row jobs of 65536 (128 MiB each) and a small LDS-resident job:
-    one persistent launch plus one launch per four-step pass; results are
-    bitwise those of the standalone kernels."""
+    """Jobs of 65536 with 300 and 100 rows (150 + 50 MiB) and a small
+    LDS-resident job.  The four-step rows run in chunks of 128 MiB (256 rows
+    of 65536), the first job's rows split over both chunks, each chunk's
+    three passes back to back: one persistent launch plus 3 per chunk.
+    Results are bitwise those of the standalone kernels."""
     import torch
     rng = np.random.default_rng(42)
     jobs = []
     reps = {}
-    for n, rows in [(65536, 256), (65536, 256), (1024, 8)]:
+    for n, rows in [(65536, 300), (65536, 100), (1024, 8)]:
         if n not in reps:
             reps[n] = ovx.Replica(n, wl.cgauss(rng, n), is_spectrum=True)
         xd = torch.randn(rows, n, dtype=torch.complex64, device="cuda")
@@ -124,7 +126,7 @@ def test_batch_large_four_step_jobs_bitwise_equal_standalone(ovx):
     for n, rows, xd, yd in jobs:
         b.add(xd, reps[n], yd)
     b.commit()
-    assert b.launches == 1 + 3
+    assert b.launches == 1 + 3 * 2
     b()
     for n, rows, xd, yd in jobs:
         alone = ovx.Fastconv(rows, n, variant=0).with_replica(xd, reps[n])
