@@ -198,9 +198,10 @@ int ovx_fastconv_exec_replica(ovx_plan plan, const void *x, long ldx, int replic
 /* No reference counterpart: the reference runs one Fftm object per shape
    (benchmarks/fastconv.cpp loops one size at a time).  A batch packs jobs of
    different n and row counts (each against a resident replica) into one
-   persistent-kernel launch over an (instance, rows) tile queue; jobs on the
-   four-step path add one launch per pass and line length (one line group
-   per workgroup).  The launches are captured once into a hipGraph at commit
+   persistent-kernel launch over an (instance, rows) tile queue; the rows of
+   jobs on the four-step path run in chunks of about 128 MiB, each chunk one
+   launch per pass and line length (one line group per workgroup), so that a
+   chunk stays in the Infinity Cache between its passes.  The launches are captured once into a hipGraph at commit
    and replayed by every exec.  Each job's result is bitwise the same as
    ovx_fastconv_exec_replica on the same job.  ovx_batch_launches returns
    the number of kernel launches per replay.  A committed batch holds the
