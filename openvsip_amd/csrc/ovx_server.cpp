@@ -35,6 +35,9 @@
 
 namespace ovx {
 
+// Batches of at most this many requests run as standalone launches.
+constexpr int kDirectMax = 4;
+
 static double now_s()
 {
   using namespace std::chrono;
@@ -387,8 +390,18 @@ void Server::scheduler()
       }
     }
     int st = OVX_OK;
-    if (cfg.plain) st = launch_plain(*s, *batch[0]);
-    else st = admit_and_launch(*s, batch);
+    if (cfg.plain) {
+      st = launch_plain(*s, *batch[0]);
+    } else if ((int)batch.size() <= kDirectMax) {
+      // a small batch (light load): the requests' standalone launches on the
+      // slot's stream cost less than replaying the slot graph (a persistent
+      // launch over every CU plus one launch per four-step queue); same
+      // device bodies, same results
+      for (Req *r : batch)
+        if ((st = launch_plain(*s, *r))) break;
+    } else {
+      st = admit_and_launch(*s, batch);
+    }
     if (st == OVX_OK) {
       hipError_t e = hipEventRecord(s->done, s->stream);
       if (e != hipSuccess) st = hip_fail(e, "event record");
