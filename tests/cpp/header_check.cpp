@@ -1,0 +1,45 @@
+// Compiles include/ovxhip/ovxhip.hpp against the C-ABI and runs the error
+// paths that need no device: each C-ABI status must surface as the exception
+// the header maps it to (unimplemented, std::invalid_argument), the way the
+// reference's backends report a declined shape or a bad argument.
+#include <cstdio>
+#include <stdexcept>
+
+#include "ovxhip/ovxhip.hpp"
+
+static int fails = 0;
+
+#define EXPECT_THROW(stmt, type)                                  \
+  do {                                                            \
+    try {                                                         \
+      stmt;                                                       \
+      std::printf("no exception: %s\n", #stmt);                   \
+      ++fails;                                                    \
+    } catch (type const &) {                                      \
+    } catch (std::exception const &e) {                           \
+      std::printf("wrong exception for %s: %s\n", #stmt, e.what()); \
+      ++fails;                                                    \
+    }                                                             \
+  } while (0)
+
+int main()
+{
+  // a bad axis is an invalid argument
+  EXPECT_THROW((ovxhip::Fftm<5, OVX_FFT_FWD>(4, 16, 1.0f)), std::invalid_argument);
+  // a prime length is declined: the dispatcher's next backend takes it
+  EXPECT_THROW((ovxhip::Fftm<OVX_ROW, OVX_FFT_FWD>(4, 19, 1.0f)), ovxhip::unimplemented);
+  EXPECT_THROW((ovxhip::Fft<OVX_FFT_INV>(68, 1.0f)), ovxhip::unimplemented);
+  // real Fftm needs an even length
+  EXPECT_THROW((ovxhip::FftmReal<OVX_ROW, OVX_FFT_FWD>(4, 15, 1.0f)), ovxhip::unimplemented);
+  // a reference longer than the input
+  EXPECT_THROW((ovxhip::Correlation(0, 10, 5)), std::invalid_argument);
+  try {
+    ovxhip::check(OVX_ERR_UNSUPPORTED);
+    ++fails;
+  } catch (ovxhip::error const &e) {
+    if (e.code() != OVX_ERR_UNSUPPORTED) ++fails;
+  }
+  if (ovx_plan_destroy(nullptr) != OVX_ERR_INVALID) ++fails;
+  std::printf("%s\n", fails ? "FAIL" : "OK");
+  return fails;
+}
