@@ -120,3 +120,60 @@ def test_server_close_with_requests_in_flight(ovx):
     torch.cuda.synchronize()
     z = torch.ones(4, device="cuda") * 2
     assert float(z.sum()) == 8.0
+
+
+def test_server_large_four_step_batch(ovx):
+    """A batch holding ~290 MB of four-step rows (three fastconv requests of
+    65536 x 120 rows and a forward Fftm of 65536 x 100 rows, so fastconv and
+    Fftm tiles share the pass queues) next to small LDS-resident requests:
+    every result is bitwise the per-request server's and matches the FP64
+    oracle on sampled rows."""
+    import torch
+    rng = np.random.default_rng(77)
+    w = wl.replica_spectrum(wl.lfm_chirp(65536, 8192), 65536)
+    rep = ovx.Replica(65536, w, is_spectrum=True)
+    w2 = wl.replica_spectrum(wl.lfm_chirp(2048, 256), 2048)
+    rep2 = ovx.Replica(2048, w2, is_spectrum=True)
+    items = [("fc", 120), ("fc", 120), ("fft", 100), ("fc", 120), ("small", 40), ("small", 64)]
+    xs = []
+    for kind, rows in items:
+        n = 2048 if kind == "small" else 65536
+        xs.append((kind, torch.from_numpy(wl.cgauss(rng, (rows, n))).cuda()))
+
+    def run(plain):
+        srv = ovx.Server(slots=3, max_jobs=64, hbm_budget_mb=16384, workspace_mb=256, plain=plain)
+        # keep every slot busy so that the requests below queue up and the
+        # next free slot takes them as one batch
+        busy_x = torch.randn((256, 65536), dtype=torch.complex64, device="cuda")
+        busy_y = [torch.empty_like(busy_x) for _ in range(3)]
+        busy = [srv.submit(ovx.fastconv_request(busy_x, rep, y)) for y in busy_y]
+        outs, tickets = [], []
+        for kind, x in xs:
+            y = torch.full_like(x, float("nan"))
+            if kind == "fc":
+                req = ovx.fastconv_request(x, rep, y)
+            elif kind == "fft":
+                req = ovx.fftm_request(x, y)
+            else:
+                req = ovx.fastconv_request(x, rep2, y)
+            tickets.append(srv.submit(req))
+            outs.append(y)
+        for t in busy + tickets:
+            srv.wait(t)
+        res = [host(y) for y in outs]
+        srv.close()
+        return res
+
+    batched = run(False)
+    plain = run(True)
+    for (kind, x), yb, yp in zip(xs, batched, plain):
+        assert np.isfinite(yb).all(), kind
+        assert np.array_equal(yb, yp), kind
+        xh = host(x)[:4]
+        if kind == "fc":
+            ref = o.fastconv64(xh, w)
+        elif kind == "fft":
+            ref = o.fftm64(xh.astype(np.complex128), o.FWD)
+        else:
+            ref = o.fastconv64(xh, w2)
+        assert o.rel_l2(yb[:4], ref) < 1e-5, kind
