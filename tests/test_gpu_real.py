@@ -1,8 +1,9 @@
 """GPU parity tests of the real <-> complex Fftm (r2c / c2r through a
 half-length complex transform) against the FP64 oracle, which follows the
 reference's FFTW r2c/c2r calls (src/ovxx/fftw/fftw.cpp:778-903) and is pinned
-against numpy.fft.rfft / irfft in tests/test_oracle.py.  Tolerance: rel-L2 <=
-1e-5 (complex<float> bar)."""
+against numpy.fft.rfft / irfft and the reference's own r2c / c2r golden files
+in tests/test_oracle.py.  Tolerance: rel-L2 <= 1e-5 (complex<float> bar); the
+golden-file tests use the reference's < -100 dB criterion."""
 import numpy as np
 import pytest
 
@@ -124,3 +125,38 @@ def test_real_fftm_refuses_aliased_input_and_output(ovx):
     st = _lib.lib().ovx_fftm_real_out_of_place(f._h, p, n + 2, 1, p, (n + 2) // 2, 1, rows, n,
                                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     assert st == _lib.OVX_ERR_UNSUPPORTED
+
+
+def _gold_real():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_ext_real.npz"))
+
+
+@pytest.mark.parametrize("k", range(1, 9))
+def test_r2c_reference_golden_vectors(ovx, k):
+    """The reference's r2c golden files tests/fft_ext/data/rcfftop_f* (lengths
+    4, 768, 1024; scales 1, 3, 1/1024), through a one-row real Fftm, at the
+    reference's own criterion (tests/fft_ext/fft_ext.cpp:85-108: < -100 dB)."""
+    import torch
+    g = _gold_real()
+    name = f"rcfftop_f{k}"
+    x, e, sc = g[name + "_in"], g[name + "_expected"], float(g[name + "_scale"])
+    f = ovx.FftmReal(1, len(x), ovx.ROW, ovx.FFT_FWD, sc)
+    y = host(f(torch.from_numpy(x[None, :].copy()).cuda()))[0]
+    say(name, len(x), sc, "db", o.diff_db(e.astype(np.complex128), y))
+    assert o.diff_db(e.astype(np.complex128), y) < -100
+
+
+@pytest.mark.parametrize("k", range(1, 8))
+def test_c2r_reference_golden_vectors(ovx, k):
+    """The reference's c2r golden files crfftop_f* (unnormalised inverse
+    times the file's scale) through a one-row real inverse Fftm."""
+    import torch
+    g = _gold_real()
+    name = f"crfftop_f{k}"
+    xh, e, sc = g[name + "_in"], g[name + "_expected"], float(g[name + "_scale"])
+    n = len(e)
+    f = ovx.FftmReal(1, n, ovx.ROW, ovx.FFT_INV, sc)
+    y = host(f(torch.from_numpy(xh[None, :].copy()).cuda()))[0]
+    say(name, n, sc, "db", o.diff_db(e.astype(np.float64), y.astype(np.float64)))
+    assert o.diff_db(e.astype(np.float64), y.astype(np.float64)) < -100

@@ -53,6 +53,35 @@ def test_fft64_matches_reference_golden(k):
     assert o.diff_db(e, y) < -100
 
 
+# ---- real<->complex golden vectors (rcfftop_f1..f8, crfftop_f1..f7) ---------
+
+def gold_real():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_ext_real.npz"))
+
+
+@pytest.mark.parametrize("k", range(1, 9))
+def test_rfftm64_matches_reference_golden(k):
+    """The real-input transform of the oracle against the reference's own
+    r2c golden files (tests/fft_ext/fft_ext.cpp:213-262, < -100 dB)."""
+    g = gold_real()
+    name = f"rcfftop_f{k}"
+    x, e, s = g[name + "_in"], g[name + "_expected"], float(g[name + "_scale"])
+    y = o.rfftm64(x[None, :], s)[0]
+    assert o.diff_db(e.astype(np.complex128), y) < -100
+
+
+@pytest.mark.parametrize("k", range(1, 8))
+def test_irfftm64_matches_reference_golden(k):
+    """The oracle's complex-to-real transform (unnormalised, times scale)
+    against the reference's c2r golden files (fft_ext.cpp:164-210)."""
+    g = gold_real()
+    name = f"crfftop_f{k}"
+    xh, e, s = g[name + "_in"], g[name + "_expected"], float(g[name + "_scale"])
+    y = o.irfftm64(xh[None, :], len(e), s)[0]
+    assert o.diff_db(e.astype(np.float64), y) < -100
+
+
 # ---- FP64 FFT against brute force and a library routine ----------------------
 
 @pytest.mark.parametrize("n", list(range(1, 65)) + [96, 100, 243, 250, 252, 768, 1000])
