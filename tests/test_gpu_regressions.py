@@ -3,9 +3,11 @@ backend: Fftm on sub-views whose rows start at an element offset inside a
 wider matrix (row stride size + gap, first column `align`), out of place and
 in place (tests/regressions/fftm_unaligned.cpp:20-118: 64 x 256 with (gap,
 align) = (0, 0), (1, 1), (1, 0)), plus the same on the four-step and
-hierarchical lengths, and a fastconv whose input, output and spectrum views
-all start at odd element offsets.  Elements outside the view stay as they
-were."""
+hierarchical lengths; a fastconv whose input, output and spectrum views all
+start at odd element offsets; the 1-D Fft on offset views and in place on a
+matrix row, and Fftm in place on a sub-view (tests/regressions/
+fft_unaligned.cpp, fft_ip_subview.cpp).  Elements outside the views stay as
+they were."""
 import numpy as np
 import pytest
 
@@ -92,3 +94,73 @@ def test_fastconv_odd_offset_views(ovx, n):
     ref = o.fastconv64(x, w)
     assert o.rel_l2(y[:, 1:n + 1], ref) < 1e-5
     assert np.all(y[:, 0] == complex(9, 9)) and np.all(y[:, n + 1:] == complex(9, 9))
+
+
+def _fft_in_place(f, t):
+    import ctypes
+    import torch
+    from openvsip_amd import _lib
+    _lib.check(_lib.lib().ovx_fft_in_place(f._h, ctypes.c_void_p(t.data_ptr()), t.stride(0), f.n,
+                                           ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+@pytest.mark.parametrize("align", [0, 1])
+def test_fft_unaligned(ovx, align):
+    """tests/regressions/fft_unaligned.cpp: a 256-point Fft of ones on a view
+    starting `align` elements into its vector, out of place and in place;
+    the transform is 256 at frequency 0."""
+    import torch
+    size = 256
+    f = ovx.Fft(size, ovx.FFT_FWD, 1.0)
+    inp = torch.ones(size + align, dtype=torch.complex64, device="cuda")
+    out = torch.zeros(size + align, dtype=torch.complex64, device="cuda")
+    f(inp[align:], out[align:])
+    assert host(out)[align] == size
+    _fft_in_place(f, inp[align:])
+    y = host(inp)
+    assert y[align] == size
+    assert np.all(np.abs(y[align + 1:]) < 1e-5 * size)
+
+
+def test_fft_in_place_on_vector_and_matrix_row(ovx):
+    """tests/regressions/fft_ip_subview.cpp:17-47: the constant (1, -2) of
+    length 64 transformed in place, as a vector and as the last row of a
+    32 x 64 matrix: (64, -128) at frequency 0, zero elsewhere; the other
+    rows of the matrix keep their values."""
+    import torch
+    rows, size = 32, 64
+    f = ovx.Fft(size, ovx.FFT_FWD, 1.0)
+    vec = torch.full((size,), complex(1, -2), dtype=torch.complex64, device="cuda")
+    mat = torch.full((rows, size), complex(5, 5), dtype=torch.complex64, device="cuda")
+    mat[rows - 1] = complex(1, -2)
+    _fft_in_place(f, vec)
+    _fft_in_place(f, mat[rows - 1])
+    for y in (host(vec), host(mat)[rows - 1]):
+        assert y[0] == complex(size, -2 * size)
+        assert np.all(np.abs(y[1:]) < 1e-5 * size)
+    assert np.all(host(mat)[: rows - 1] == complex(5, 5))
+
+
+def test_fftm_in_place_on_subview(ovx):
+    """tests/regressions/fft_ip_subview.cpp:50-96: Fftm (scale 1/cols) in
+    place on a 16 x 64 matrix and on the same-sized sub-view [8:24, 32:96] of
+    a 32 x 128 matrix, row r holding the constant r: frequency 0 of row r is
+    exactly r in both, and the big matrix outside the view keeps -101."""
+    import torch
+    rows, cols = 16, 64
+    f = ovx.Fftm(rows, cols, ovx.ROW, ovx.FFT_FWD, 1.0 / cols)
+    inout = torch.zeros((rows, cols), dtype=torch.complex64, device="cuda")
+    big = torch.full((2 * rows, 2 * cols), complex(-101, 0), dtype=torch.complex64, device="cuda")
+    view = big[rows // 2:rows // 2 + rows, cols // 2:cols // 2 + cols]
+    for r in range(rows):
+        inout[r] = r
+        view[r] = r
+    f.in_place(inout)
+    f.in_place(view)
+    a, b = host(inout), host(big)
+    v = b[rows // 2:rows // 2 + rows, cols // 2:cols // 2 + cols]
+    for r in range(rows):
+        assert a[r, 0] == r and v[r, 0] == r, r
+    mask = np.ones(b.shape, bool)
+    mask[rows // 2:rows // 2 + rows, cols // 2:cols // 2 + cols] = False
+    assert np.all(b[mask] == complex(-101, 0))
