@@ -382,3 +382,25 @@ def test_autotune_records_and_plans_use_it(ovx, tmp_path, monkeypatch):
     fc = ovx.Fastconv(300, 4096)
     y = host(fc(dev(x), dev(w)))
     assert o.rel_l2(y, o.fastconv64(x, w)) < REL_L2
+
+
+# the reference's own size list (tests/correlation.cpp:225-289: corr_cover and
+# impl_corr_cover), each with its three input patterns (:68-82), at its
+# threshold (-100 dB, :20-29)
+CORR_COVER = [(8, 8), (1, 128), (7, 128), (8, 128), (9, 128), (7, 127), (8, 127), (9, 127), (7, 129),
+              (8, 129), (9, 129), (32, 128), (64, 128), (96, 128), (12, 96), (12, 97), (11, 97), (12, 98)]
+
+
+@pytest.mark.parametrize("support", ["full", "same", "min"])
+@pytest.mark.parametrize("bias", ["biased", "unbiased"])
+@pytest.mark.parametrize("m,n", CORR_COVER)
+def test_correlation_reference_cases(ovx, support, bias, m, n):
+    rng = np.random.default_rng(1000 + 7 * m + n)
+    ramp = np.arange(n, dtype=np.float64).astype(np.complex64)
+    c = ovx.Correlation(support, m, n)
+    for ref, x in ((np.ones(m, np.complex64), ramp),
+                   (cg(rng, m), ramp),
+                   (cg(rng, m), cg(rng, n))):
+        y = host(c(bias, dev(ref), dev(x)))
+        exp = o.corr64(bias, support, ref, x)
+        assert o.diff_db(exp, y) < DB, (support, bias, m, n)
