@@ -404,3 +404,27 @@ def test_correlation_reference_cases(ovx, support, bias, m, n):
         y = host(c(bias, dev(ref), dev(x)))
         exp = o.corr64(bias, support, ref, x)
         assert o.diff_db(exp, y) < DB, (support, bias, m, n)
+
+
+def _conv_cover():
+    """(symmetry, input size, coefficient count, decimation) of the
+    reference's cases() (tests/convolution/convolution.hpp:453-489)."""
+    c = [("nonsym", 8, 8, 1), ("nonsym", 5, 5, 1), ("sym_even_len_even", 8, 4, 1), ("sym_even_len_odd", 7, 4, 1)]
+    c += [("nonsym", 5, 4, d) for d in (1, 2, 3, 4)]
+    for size in (32, 128, 512):
+        c += [("nonsym", size, 8, 1), ("nonsym", 2 * size, 7, 2), ("nonsym", size + 4, 6, 3),
+              ("sym_even_len_even", size, 5, 1), ("sym_even_len_even", size + 1, 6, 2),
+              ("sym_even_len_odd", size, 4, 1), ("sym_even_len_odd", size + 3, 3, 2)]
+    return c
+
+
+@pytest.mark.parametrize("support", ["full", "same", "min"])
+@pytest.mark.parametrize("symmetry,n,m,d", _conv_cover())
+def test_convolution_reference_cases(ovx, symmetry, n, m, d, support):
+    rng = np.random.default_rng(n * 31 + m * 7 + d)
+    coeff, x = cg(rng, m), cg(rng, n)
+    conv = ovx.Convolution(coeff, symmetry, support, n, d)
+    y = host(conv(dev(x)))
+    exp = o.conv64(support, o.conv_kernel(symmetry, coeff), x, d)
+    assert y.shape == exp.shape
+    assert o.diff_db(exp, y) < DB, (symmetry, n, m, d, support)
