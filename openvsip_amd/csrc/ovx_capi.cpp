@@ -399,21 +399,41 @@ static int check_plan(ovx_plan p, int kind)
 static bool is_dir(int d) { return d == OVX_FFT_FWD || d == OVX_FFT_INV; }
 static int sign_of(int d) { return d == OVX_FFT_FWD ? -1 : +1; }
 
-// Host-pointer staging: copy in, run `body` on device pointers, copy out,
-// wait.  in_bytes/out_bytes cover the strided extents.
-template <class F>
-static int with_staging(ovx_plan p, const void *in, size_t in_bytes, void *out, size_t out_bytes, hipStream_t s,
-                        F body)
+// `count` lines of `width` bytes, `pitch` bytes apart, in a host buffer and
+// at the same layout in its device staging copy.
+struct HostLines {
+  size_t count, pitch, width;
+  size_t extent() const { return count ? (count - 1) * pitch + width : 0; }
+};
+
+static HostLines host_lines(long count, long ld, long len, size_t elem)
 {
-  int st = p->stage_in.alloc(in_bytes);
+  const size_t w = (size_t)len * elem;
+  return {(size_t)count, count > 1 ? (size_t)ld * elem : w, w};
+}
+
+// Only the lines' own bytes move: the gaps between lines stay as they are
+// on the host, and uninitialised staging memory is never copied back.
+static hipError_t copy_lines(void *dst, const void *src, HostLines g, hipMemcpyKind kind, hipStream_t s)
+{
+  if (g.count <= 1 || g.pitch == g.width) return hipMemcpyAsync(dst, src, g.extent(), kind, s);
+  return hipMemcpy2DAsync(dst, g.pitch, src, g.pitch, g.width, g.count, kind, s);
+}
+
+// Host-pointer staging: copy the input lines in, run `body` on device
+// pointers, copy the output lines out, wait.
+template <class F>
+static int with_staging(ovx_plan p, const void *in, HostLines gi, void *out, HostLines go, hipStream_t s, F body)
+{
+  int st = p->stage_in.alloc(gi.extent());
   if (st) return st;
-  st = p->stage_out.alloc(out_bytes);
+  st = p->stage_out.alloc(go.extent());
   if (st) return st;
-  hipError_t e = hipMemcpyAsync(p->stage_in.p, in, in_bytes, hipMemcpyHostToDevice, s);
+  hipError_t e = copy_lines(p->stage_in.p, in, gi, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "H2D copy");
   st = body(p->stage_in.p, p->stage_out.p);
   if (st) return st;
-  e = hipMemcpyAsync(out, p->stage_out.p, out_bytes, hipMemcpyDeviceToHost, s);
+  e = copy_lines(out, p->stage_out.p, go, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return hip_fail(e, "D2H copy");
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_fail(e, "stream synchronize");
@@ -509,11 +529,11 @@ static int fftm_geometry(ovx_plan p, long s0, long s1, long rows, long cols, lon
   return OVX_OK;
 }
 
-extern "C" int ovx_fftm_out_of_place(ovx_plan p, const void *in, long in_s0, long in_s1, void *out, long out_s0,
-                                     long out_s1, long rows, long cols, void *stream)
+// The Fftm and Fft entry points' common body (the plan's kind already checked).
+static int fftm_exec(ovx_plan p, const void *in, long in_s0, long in_s1, void *out, long out_s0, long out_s1,
+                     long rows, long cols, void *stream)
 {
-  int st = check_plan(p, PLAN_FFTM);
-  if (st) return st;
+  int st;
   long n, lines, ldi, ldo;
   if ((st = fftm_geometry(p, in_s0, in_s1, rows, cols, &n, &lines, &ldi))) return st;
   if ((st = fftm_geometry(p, out_s0, out_s1, rows, cols, &n, &lines, &ldo))) return st;
@@ -522,12 +542,20 @@ extern "C" int ovx_fftm_out_of_place(ovx_plan p, const void *in, long in_s0, lon
   hipStream_t s = (hipStream_t)stream;
   int dir = sign_of(p->dir);
   if (p->flags & OVX_FLAG_HOST_POINTERS) {
-    size_t ib = ((size_t)(lines - 1) * ldi + n) * 8, ob = ((size_t)(lines - 1) * ldo + n) * 8;
-    return with_staging(p, in, ib, out, ob, s, [&](void *din, void *dout) {
+    const HostLines gi = host_lines(lines, ldi, n, 8), go = host_lines(lines, ldo, n, 8);
+    return with_staging(p, in, gi, out, go, s, [&](void *din, void *dout) {
       return p->line.transform(dir, (const float2 *)din, ldi, (float2 *)dout, ldo, lines, p->scale, s);
     });
   }
   return p->line.transform(dir, (const float2 *)in, ldi, (float2 *)out, ldo, lines, p->scale, s);
+}
+
+extern "C" int ovx_fftm_out_of_place(ovx_plan p, const void *in, long in_s0, long in_s1, void *out, long out_s0,
+                                     long out_s1, long rows, long cols, void *stream)
+{
+  int st = check_plan(p, PLAN_FFTM);
+  if (st) return st;
+  return fftm_exec(p, in, in_s0, in_s1, out, out_s0, out_s1, rows, cols, stream);
 }
 
 extern "C" int ovx_fftm_in_place(ovx_plan p, void *inout, long s0, long s1, long rows, long cols, void *stream)
@@ -552,10 +580,7 @@ extern "C" int ovx_fft_out_of_place(ovx_plan p, const void *in, long in_stride, 
   if (st) return st;
   if (n != p->cols) return fail(OVX_ERR_INVALID, "length %ld does not match the plan's %ld", n, p->cols);
   if ((in_stride != 1 || out_stride != 1) && n > 1) return fail(OVX_ERR_UNSUPPORTED, "strides must be 1");
-  p->kind = PLAN_FFTM;
-  st = ovx_fftm_out_of_place(p, in, n, 1, out, n, 1, 1, n, stream);
-  p->kind = PLAN_FFT;
-  return st;
+  return fftm_exec(p, in, n, 1, out, n, 1, 1, n, stream);
 }
 
 extern "C" int ovx_fft_in_place(ovx_plan p, void *inout, long stride, long n, void *stream)
@@ -648,8 +673,8 @@ extern "C" int ovx_fftm_real_out_of_place(ovx_plan p, const void *in, long in_s0
     return p->line.transform(+1, (const float2 *)dout, ldz, (float2 *)dout, ldz, lines, p->scale, s);
   };
   if (p->flags & OVX_FLAG_HOST_POINTERS) {
-    const size_t rb = ((size_t)(lines - 1) * ldr + n) * 4, cb = ((size_t)(lines - 1) * ldc + h + 1) * 8;
-    return with_staging(p, in, fwd ? rb : cb, out, fwd ? cb : rb, s, run);
+    const HostLines gr = host_lines(lines, ldr, n, 4), gc = host_lines(lines, ldc, h + 1, 8);
+    return with_staging(p, in, fwd ? gr : gc, out, fwd ? gc : gr, s, run);
   }
   return run(in, out);
 }
@@ -712,13 +737,11 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
   }
   if (p->flags & OVX_FLAG_HOST_POINTERS) {
     // stage x and w together, compute into stage_out
-    size_t xb = ((size_t)(rows - 1) * ldx + n) * 8;
-    size_t wb = ldw ? ((size_t)(rows - 1) * ldw + n) * 8 : (size_t)n * 8;
-    size_t yb = ((size_t)(rows - 1) * ldy + n) * 8;
-    if ((st = p->stage_w.alloc(wb))) return st;
-    hipError_t e = hipMemcpyAsync(p->stage_w.p, w, wb, hipMemcpyHostToDevice, s);
+    const HostLines gw = host_lines(ldw ? rows : 1, ldw, n, 8);
+    if ((st = p->stage_w.alloc(gw.extent()))) return st;
+    hipError_t e = copy_lines(p->stage_w.p, w, gw, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "H2D copy");
-    return with_staging(p, x, xb, y, yb, s, [&](void *dx, void *dy) {
+    return with_staging(p, x, host_lines(rows, ldx, n, 8), y, host_lines(rows, ldy, n, 8), s, [&](void *dx, void *dy) {
       return p->line.fastconv((const float2 *)dx, ldx, (const float2 *)p->stage_w.p, ldw, (float2 *)dy, ldy, rows,
                               p->scale, s);
     });
