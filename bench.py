@@ -44,6 +44,9 @@ def parse():
                     help="comma-separated N: keep only the batch's jobs of these lengths (per-length breakdown)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16x3 MFMA variant measurement")
+    ap.add_argument("--no-drop-in", action="store_true",
+                    help="skip the host-pointer (VSIPL++ plugin) vs resident-storage measurement")
+    ap.add_argument("--drop-in-calls", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device copy of the workload's bytes instead (reference for HBM counters)")
     ap.add_argument("--serve-rates", default="2000,5000,10000,20000", help="offered rates, jobs/s")
@@ -114,6 +117,84 @@ def measured_traffic(plan_name, rows, n):
         if rec.get("plan_name") == plan_name and rec.get("rows") == rows and rec.get("n") == n:
             return rec["traffic_bytes"]
     return None
+
+
+def drop_in_section(args, x, w):
+    """The VSIPL++ drop-in path at the headline shape.  OpenVSIP's DDA hands
+    the backend host pointers (ordinary pageable arrays), so a plan made with
+    OVX_FLAG_HOST_POINTERS copies X in and Y out on every call; with the
+    block's data in an ovx_storage the same calls find X already resident and
+    copy nothing until the host reads Y."""
+    import numpy as np
+    import torch
+
+    from openvsip_amd import _lib
+    from openvsip_amd.api import Storage
+    lib = _lib.lib()
+    R, N = x.shape
+    x = np.ascontiguousarray(x, dtype=np.complex64)
+    w = np.ascontiguousarray(w, dtype=np.complex64)
+    calls = args.drop_in_calls
+    y = np.empty_like(x)
+    hp = ctypes.c_void_p()
+    _lib.check(lib.ovx_fastconv_plan_create(ctypes.byref(hp), R, N, 1.0 / N, _lib.FLAG_HOST_POINTERS))
+    dp = ctypes.c_void_p()
+    _lib.check(lib.ovx_fastconv_plan_create(ctypes.byref(dp), R, N, 1.0 / N, 0))
+    try:
+        px, pw, py = (ctypes.c_void_p(a.ctypes.data) for a in (x, w, y))
+        for _ in range(2):
+            _lib.check(lib.ovx_fastconv_exec(hp, px, N, pw, 0, py, N, R, None))
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            _lib.check(lib.ovx_fastconv_exec(hp, px, N, pw, 0, py, N, R, None))
+        host_ms = (time.perf_counter() - t0) / calls * 1e3
+        y_host = y.copy()
+
+        sx, sw, sy = Storage(x.nbytes, x), Storage(w.nbytes, w), Storage(y.nbytes, y)
+
+        def resident_call():
+            dx, dw = sx.ptr(Storage.DEVICE, Storage.READ), sw.ptr(Storage.DEVICE, Storage.READ)
+            dy = sy.ptr(Storage.DEVICE, Storage.WRITE)
+            _lib.check(lib.ovx_fastconv_exec(dp, dx, N, dw, 0, dy, N, R, None))
+
+        t0 = time.perf_counter()
+        resident_call()  # pays the one H2D of X and W
+        torch.cuda.synchronize()
+        first_ms = (time.perf_counter() - t0) * 1e3
+        for _ in range(args.warmup):
+            resident_call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            resident_call()
+        torch.cuda.synchronize()
+        resident_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        # Y read on the host after every call: one D2H per call
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            resident_call()
+            sy.ptr(Storage.HOST, Storage.READ)
+        out_ms = (time.perf_counter() - t0) / calls * 1e3
+        h2d, d2h = sx.info()[1], sy.info()[2]
+        same = bool(np.array_equal(y, y_host))
+        for s in (sx, sw, sy):
+            s.close()
+    finally:
+        lib.ovx_plan_destroy(hp)
+        lib.ovx_plan_destroy(dp)
+    moved = 2 * R * N * 8
+    return {
+        "what": "fastconv 256x16384 as the VSIPL++ plugin calls it: host pointers (pageable arrays, "
+                "copied in and out every call) vs the same data kept in ovx_storage",
+        "host_pointers": {"ms_per_call": host_ms, "rows_per_s": R / (host_ms * 1e-3),
+                          "host_link_GBs": moved / (host_ms * 1e-3) / 1e9, "calls": calls},
+        "storage_resident": {"first_call_ms": first_ms, "ms_per_call": resident_ms,
+                             "rows_per_s": R / (resident_ms * 1e-3), "calls": args.steps,
+                             "h2d_bytes_total": h2d},
+        "storage_y_to_host_each_call": {"ms_per_call": out_ms, "rows_per_s": R / (out_ms * 1e-3),
+                                        "d2h_bytes_total": d2h, "calls": calls},
+        "y_bitwise_equal_across_paths": same,
+    }
 
 
 def precision_section(args, x, w, xd, fp32_ms, fp32_rel):
@@ -440,6 +521,8 @@ def main():
     }
     if not args.no_precision and N == 16384:
         out["precision_bf16x3"] = precision_section(args, x, w, xd, ms, rel)
+    if not args.no_drop_in:
+        out["drop_in"] = drop_in_section(args, x, w)
     if not args.no_batch:
         out["batch"] = batch_section(args)
     if not args.no_serve:

@@ -194,6 +194,44 @@ int ovx_replica_spectrum(int id, const void **spectrum, long *n);
 int ovx_fastconv_exec_replica(ovx_plan plan, const void *x, long ldx, int replica,
                               void *y, long ldy, long rows, void *stream);
 
+/* ---- storage: a block's data in host memory and in HBM ----------------- */
+/* The reference's storage_manager (src/ovxx/storage/manager.hpp:207-360)
+   keeps a block's data in host memory plus, when configured, a device copy
+   (src/ovxx/opencl/storage.hpp; commented out of the manager), each valid or
+   stale.  ovx_storage is that manager with a HIP location: a block's data
+   stays in HBM across calls and crosses the host link only when the other
+   location asks for it.  Not safe for concurrent calls, like a plan. */
+typedef struct ovx_storage_s *ovx_storage;
+#define OVX_LOC_HOST 0
+#define OVX_LOC_DEVICE 1
+#define OVX_ACCESS_READ 1       /* const ptr(where): sync the location      */
+#define OVX_ACCESS_WRITE 2      /* contents will be overwritten (dda::out):
+                                   no copy; every other location goes stale */
+#define OVX_ACCESS_READ_WRITE 3 /* ptr(where): sync, then the others go
+                                   stale (manager.hpp:247-263)             */
+/* `bytes` of storage.  With user_host != NULL the application's array is the
+   host location and holds the data (user storage, admitted); otherwise the
+   library allocates pinned host memory when the host first asks. */
+int ovx_storage_create(ovx_storage *storage, size_t bytes, void *user_host);
+int ovx_storage_destroy(ovx_storage storage);
+/* A pointer valid in `where` for `access`.  A stale location is refreshed
+   first with a copy queued on `stream`: the device pointer is ready for work
+   queued on that stream afterwards; a host pointer is returned after its
+   copy has landed.  Device work that wrote the data must have been queued
+   on `stream` (or be complete) before a host access. */
+int ovx_storage_ptr(ovx_storage storage, int where, int access, void **ptr,
+                    void *stream);
+/* User storage (manager.hpp:207-243): release brings the host array up to
+   date and hands it to the application (pointer requests then fail with
+   OVX_ERR_STATE); admit takes it back -- with update != 0 the application may
+   have changed it, so the device copy is stale. */
+int ovx_storage_release(ovx_storage storage, void *stream);
+int ovx_storage_admit(ovx_storage storage, int update);
+/* Which locations are valid (bit 1 << OVX_LOC_*) and the bytes copied each
+   way so far. */
+int ovx_storage_info(ovx_storage storage, unsigned *valid, uint64_t *h2d_bytes,
+                     uint64_t *d2h_bytes);
+
 /* ---- batches of uneven fastconv jobs ------------------------------------ */
 /* No reference counterpart: the reference runs one Fftm object per shape
    (benchmarks/fastconv.cpp loops one size at a time).  A batch packs jobs of

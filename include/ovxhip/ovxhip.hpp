@@ -222,6 +222,49 @@ private:
   plan_holder p_;
 };
 
+// A block's data kept in host memory and in HBM (storage_manager,
+// src/ovxx/storage/manager.hpp:207-360, with a HIP device location).  T is
+// the element type; `user` (optional) is the application's array, admitted.
+template <typename T>
+class Storage {
+public:
+  explicit Storage(long size, T *user = nullptr) : size_(size)
+  {
+    check(ovx_storage_create(&h_, (size_t)size * sizeof(T), user));
+  }
+  Storage(Storage const &) = delete;
+  Storage &operator=(Storage const &) = delete;
+  ~Storage() { ovx_storage_destroy(h_); }
+  long size() const { return size_; }
+  // ptr(where) and const ptr(where) of the manager, plus write-only access
+  T *ptr(int where, int access = OVX_ACCESS_READ_WRITE, void *stream = nullptr)
+  {
+    void *p = nullptr;
+    check(ovx_storage_ptr(h_, where, access, &p, stream));
+    return static_cast<T *>(p);
+  }
+  T const *ptr(int where, void *stream = nullptr) const
+  {
+    void *p = nullptr;
+    check(ovx_storage_ptr(h_, where, OVX_ACCESS_READ, &p, stream));
+    return static_cast<T const *>(p);
+  }
+  T *device(int access = OVX_ACCESS_READ_WRITE, void *stream = nullptr) { return ptr(OVX_LOC_DEVICE, access, stream); }
+  T *host(int access = OVX_ACCESS_READ_WRITE, void *stream = nullptr) { return ptr(OVX_LOC_HOST, access, stream); }
+  void release(void *stream = nullptr) { check(ovx_storage_release(h_, stream)); }
+  void admit(bool update = true) { check(ovx_storage_admit(h_, update ? 1 : 0)); }
+  unsigned valid() const
+  {
+    unsigned v = 0;
+    check(ovx_storage_info(h_, &v, nullptr, nullptr));
+    return v;
+  }
+
+private:
+  long size_;
+  ovx_storage h_ = nullptr;
+};
+
 } // namespace ovxhip
 
 #endif

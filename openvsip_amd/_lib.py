@@ -25,6 +25,11 @@ ROW = 0
 COL = 1
 FLAG_HOST_POINTERS = 0x1
 FLAG_BF16X3 = 0x2
+LOC_HOST = 0
+LOC_DEVICE = 1
+ACCESS_READ = 1
+ACCESS_WRITE = 2
+ACCESS_READ_WRITE = 3
 
 
 def FLAG_VARIANT(v):
@@ -64,6 +69,13 @@ SIGNATURES = {
     "ovx_replica_destroy": (_I, [_I]),
     "ovx_replica_spectrum": (_I, [_I, _PP, ctypes.POINTER(_L)]),
     "ovx_fastconv_exec_replica": (_I, [_P, _P, _L, _I, _P, _L, _L, _P]),
+    "ovx_storage_create": (_I, [_PP, ctypes.c_size_t, _P]),
+    "ovx_storage_destroy": (_I, [_P]),
+    "ovx_storage_ptr": (_I, [_P, _I, _I, _PP, _P]),
+    "ovx_storage_release": (_I, [_P, _P]),
+    "ovx_storage_admit": (_I, [_P, _I]),
+    "ovx_storage_info": (_I, [_P, ctypes.POINTER(_U), ctypes.POINTER(ctypes.c_uint64),
+                              ctypes.POINTER(ctypes.c_uint64)]),
     "ovx_batch_create": (_I, [_PP]),
     "ovx_batch_add_fastconv": (_I, [_P, _L, _L, _P, _L, _I, _P, _L]),
     "ovx_batch_commit": (_I, [_P, _P]),

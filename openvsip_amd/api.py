@@ -8,6 +8,8 @@ object of the same name (paths relative to the OpenVSIP tree):
   Fastconv     inv_fftm(vmmul<0>(w, fwd_fftm(x)))  tests/fastconv.cpp:181-213
   Correlation  vsip::Correlation     src/vsip/impl/signal/corr.hpp:55-122
   Convolution  vsip::Convolution     src/vsip/impl/signal/conv.hpp:55-108
+  Storage      ovxx::storage_manager src/ovxx/storage/manager.hpp:207-360
+                                     (host location + a HIP device location)
 
 Data are torch complex64 tensors on the GPU (torch provides device memory and
 streams only); work is enqueued on torch's current stream.  Every call goes
@@ -256,6 +258,53 @@ class Replica:
 
 
 _hip_lib = None
+
+
+class Storage:
+    """A block's data in host memory and in HBM (ovx_storage): each location
+    is valid or stale and a pointer request copies only into a stale one.
+    `user` (a numpy array) becomes the host location when given; otherwise
+    the library allocates pinned host memory on first host use.  Pointers
+    are returned as ctypes.c_void_p for the C-ABI entry points."""
+
+    HOST, DEVICE = _lib.LOC_HOST, _lib.LOC_DEVICE
+    READ, WRITE, READ_WRITE = _lib.ACCESS_READ, _lib.ACCESS_WRITE, _lib.ACCESS_READ_WRITE
+
+    def __init__(self, nbytes, user=None):
+        self._h = ctypes.c_void_p()
+        self._user = user  # the array must outlive the storage
+        if user is not None and user.nbytes < nbytes:
+            raise ValueError("user array smaller than the storage")
+        check(lib().ovx_storage_create(ctypes.byref(self._h), nbytes,
+                                       None if user is None else ctypes.c_void_p(user.ctypes.data)))
+
+    def ptr(self, where, access, stream=None):
+        p = ctypes.c_void_p()
+        check(lib().ovx_storage_ptr(self._h, where, access, ctypes.byref(p), stream))
+        return p
+
+    def release(self, stream=None):
+        check(lib().ovx_storage_release(self._h, stream))
+
+    def admit(self, update=True):
+        check(lib().ovx_storage_admit(self._h, int(bool(update))))
+
+    def info(self):
+        """(valid-location bits, bytes copied host->device, device->host)"""
+        v, a, b = ctypes.c_uint(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().ovx_storage_info(self._h, ctypes.byref(v), ctypes.byref(a), ctypes.byref(b)))
+        return v.value, a.value, b.value
+
+    def close(self):
+        if self._h:
+            lib().ovx_storage_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _hip():

@@ -40,6 +40,18 @@ int main()
     if (e.code() != OVX_ERR_UNSUPPORTED) ++fails;
   }
   if (ovx_plan_destroy(nullptr) != OVX_ERR_INVALID) ++fails;
+  // user storage on the host needs no device: the application's array is
+  // the host location, valid, and released / admitted without a copy
+  {
+    ovxhip::cscalar a[8] = {};
+    ovxhip::Storage<ovxhip::cscalar> s(8, a);
+    if (s.host(OVX_ACCESS_READ) != a || s.valid() != 1u << OVX_LOC_HOST) ++fails;
+    s.release();
+    EXPECT_THROW(s.host(), ovxhip::error);
+    s.admit(true);
+    if (s.host(OVX_ACCESS_WRITE) != a) ++fails;
+    EXPECT_THROW(s.ptr(7), std::invalid_argument);
+  }
   std::printf("%s\n", fails ? "FAIL" : "OK");
   return fails;
 }

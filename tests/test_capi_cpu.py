@@ -69,3 +69,29 @@ def test_invalid_arguments_return_status_not_crash():
     assert lib.ovx_fftm_out_of_place(None, None, 0, 0, None, 0, 0, 0, 0, None) == _lib.OVX_ERR_INVALID
     with pytest.raises(_lib.OvxInvalid):
         _lib.check(lib.ovx_plan_destroy(None))
+
+
+def test_storage_host_side_needs_no_device():
+    """User storage lives in the application's array: host access, release
+    and admit involve no copy and no device (storage_manager's admit /
+    release, src/ovxx/storage/manager.hpp:207-243)."""
+    import numpy as np
+    from openvsip_amd.api import Storage
+    a = np.arange(16, dtype=np.complex64)
+    s = Storage(a.nbytes, a)
+    host = 1 << _lib.LOC_HOST
+    assert s.ptr(Storage.HOST, Storage.READ).value == a.ctypes.data
+    assert s.info() == (host, 0, 0)
+    s.release()
+    with pytest.raises(_lib.OvxError) as e:
+        s.ptr(Storage.HOST, Storage.READ)
+    assert e.value.code == _lib.OVX_ERR_STATE
+    s.admit(update=True)
+    assert s.ptr(Storage.HOST, Storage.WRITE).value == a.ctypes.data
+    assert s.info() == (host, 0, 0)
+    with pytest.raises(ValueError):
+        Storage(a.nbytes + 8, a)
+    lib = _lib.lib()
+    assert lib.ovx_storage_create(None, 8, None) == _lib.OVX_ERR_INVALID
+    assert lib.ovx_storage_ptr(None, 0, 1, None, None) == _lib.OVX_ERR_INVALID
+    assert lib.ovx_storage_info(None, None, None, None) == _lib.OVX_ERR_INVALID
