@@ -194,6 +194,32 @@ int ovx_replica_spectrum(int id, const void **spectrum, long *n);
 int ovx_fastconv_exec_replica(ovx_plan plan, const void *x, long ldx, int replica,
                               void *y, long ldy, long rows, void *stream);
 
+/* ---- split-complex storage --------------------------------------------- */
+/* The split overloads of fftm_backend / fft_backend (backend.hpp:741-783,
+   131-204; the FFTW backend's split DFTs, fftw.cpp:945-1001): real and
+   imaginary parts in two float arrays with the same strides (OpenVSIP's
+   split_complex format), on the plans of the interleaved entry points.
+   Results equal the interleaved call's on the same values bit for bit: the
+   lines are interleaved into the plan's workspace, transformed there and
+   split back (two extra passes over the data). */
+int ovx_fftm_split_out_of_place(ovx_plan plan, const void *in_re,
+                                const void *in_im, long in_s0, long in_s1,
+                                void *out_re, void *out_im, long out_s0,
+                                long out_s1, long rows, long cols, void *stream);
+int ovx_fftm_split_in_place(ovx_plan plan, void *re, void *im, long s0, long s1,
+                            long rows, long cols, void *stream);
+int ovx_fft_split_out_of_place(ovx_plan plan, const void *in_re, const void *in_im,
+                               long in_stride, void *out_re, void *out_im,
+                               long out_stride, long n, void *stream);
+int ovx_fft_split_in_place(ovx_plan plan, void *re, void *im, long stride, long n,
+                           void *stream);
+/* ovx_fastconv_exec on split X, W and Y (W one vector when ldw == 0).  Not
+   for OVX_FLAG_BF16X3 plans. */
+int ovx_fastconv_split_exec(ovx_plan plan, const void *x_re, const void *x_im,
+                            long ldx, const void *w_re, const void *w_im, long ldw,
+                            void *y_re, void *y_im, long ldy, long rows,
+                            void *stream);
+
 /* ---- storage: a block's data in host memory and in HBM ----------------- */
 /* The reference's storage_manager (src/ovxx/storage/manager.hpp:207-360)
    keeps a block's data in host memory plus, when configured, a device copy

@@ -16,6 +16,7 @@
 
 #include <complex>
 #include <new>
+#include <utility>
 #include <stdexcept>
 #include <string>
 
@@ -50,6 +51,10 @@ inline void check(int status)
 }
 
 typedef std::complex<float> cscalar;
+// split-complex storage: (real parts, imaginary parts), as the reference's
+// std::pair<T*, T*> overloads (src/ovxx/signal/fft/backend.hpp:741-783)
+typedef std::pair<float *, float *> split_ptr;
+typedef std::pair<float const *, float const *> const_split_ptr;
 
 // Owns one plan handle.
 class plan_holder {
@@ -89,6 +94,16 @@ public:
   void in_place(cscalar *inout, long s0, long s1, long rows, long cols, void *stream = nullptr)
   {
     check(ovx_fftm_in_place(p_.get(), inout, s0, s1, rows, cols, stream));
+  }
+  void out_of_place(const_split_ptr in, long in_s0, long in_s1, split_ptr out, long out_s0, long out_s1, long rows,
+                    long cols, void *stream = nullptr)
+  {
+    check(ovx_fftm_split_out_of_place(p_.get(), in.first, in.second, in_s0, in_s1, out.first, out.second, out_s0,
+                                      out_s1, rows, cols, stream));
+  }
+  void in_place(split_ptr inout, long s0, long s1, long rows, long cols, void *stream = nullptr)
+  {
+    check(ovx_fftm_split_in_place(p_.get(), inout.first, inout.second, s0, s1, rows, cols, stream));
   }
   // dense row-major (A = row) or column-major (A = col) convenience call
   void operator()(cscalar const *in, cscalar *out, void *stream = nullptr)
@@ -141,6 +156,14 @@ public:
   {
     check(ovx_fft_in_place(p_.get(), inout, s, n_, stream));
   }
+  void out_of_place(const_split_ptr in, long in_s, split_ptr out, long out_s, void *stream = nullptr)
+  {
+    check(ovx_fft_split_out_of_place(p_.get(), in.first, in.second, in_s, out.first, out.second, out_s, n_, stream));
+  }
+  void in_place(split_ptr inout, long s, void *stream = nullptr)
+  {
+    check(ovx_fft_split_in_place(p_.get(), inout.first, inout.second, s, n_, stream));
+  }
   char const *name() const { return p_.name(); }
 
 private:
@@ -165,6 +188,12 @@ public:
   void operator()(cscalar const *x, cscalar const *w, cscalar *y, void *stream = nullptr)
   {
     (*this)(x, n_, w, 0, y, n_, rows_, stream);
+  }
+  void operator()(const_split_ptr x, long ldx, const_split_ptr w, long ldw, split_ptr y, long ldy, long rows,
+                  void *stream = nullptr)
+  {
+    check(ovx_fastconv_split_exec(p_.get(), x.first, x.second, ldx, w.first, w.second, ldw, y.first, y.second, ldy,
+                                  rows, stream));
   }
   char const *name() const { return p_.name(); }
 

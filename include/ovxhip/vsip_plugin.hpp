@@ -17,6 +17,11 @@
 // through device buffers); an application that keeps its data on the device
 // calls include/ovxhip.h directly.
 //
+// The complex Fft/Fftm backends ask DDA for the storage format of OpenVSIP's
+// dense blocks (interleaved, or split when OpenVSIP is configured with split
+// complex storage), so neither format is converted on the host; split data
+// take the library's split entry points.
+//
 // This header compiles only against an OpenVSIP tree; it is not built by this
 // repository's tests.
 #ifndef OVXHIP_VSIP_PLUGIN_HPP
@@ -45,6 +50,7 @@ typedef complex<float> ctype;
 template <int A, int D>
 class fftm : public signal::fft::fftm_backend<ctype, ctype, A, D>
 {
+  typedef signal::fft::fftm_backend<ctype, ctype, A, D> base_type;
   static int const dir = D == fft_fwd ? OVX_FFT_FWD : OVX_FFT_INV;
   static int const axis = A == vsip::col ? OVX_COL : OVX_ROW;
 
@@ -55,17 +61,40 @@ public:
   virtual char const *name() { return "ovxhip::fftm"; }
   // the kernels scale: workspace.hpp:474 skips its separate pass
   virtual bool supports_scale() { return true; }
-  // default query_layout (dense, unit stride along the transform axis,
-  // interleaved complex) is what the kernels take
+  // dense, unit stride along the transform axis (the default), in the
+  // storage format OpenVSIP's dense blocks use -- interleaved or split, as
+  // configured -- so that DDA converts nothing, as the FFTW backend asks
+  // (src/ovxx/fftw/fft.cpp:24, fftw.cpp:298-307)
+  virtual void query_layout(Rt_layout<2> &rtl_inout)
+  {
+    base_type::query_layout(rtl_inout);
+    rtl_inout.storage_format = vsip::Dense<1, ctype>::storage_format;
+  }
+  virtual void query_layout(Rt_layout<2> &rtl_in, Rt_layout<2> &rtl_out)
+  {
+    base_type::query_layout(rtl_in, rtl_out);
+    rtl_in.storage_format = rtl_out.storage_format = vsip::Dense<1, ctype>::storage_format;
+  }
   virtual void in_place(ctype *inout, stride_type s0, stride_type s1, length_type rows, length_type cols)
   {
     plan_.in_place(reinterpret_cast<ovxhip::cscalar *>(inout), s0, s1, rows, cols);
+  }
+  virtual void in_place(std::pair<float *, float *> inout, stride_type s0, stride_type s1, length_type rows,
+                        length_type cols)
+  {
+    plan_.in_place(inout, s0, s1, rows, cols);
   }
   virtual void out_of_place(ctype *in, stride_type in_s0, stride_type in_s1, ctype *out, stride_type out_s0,
                             stride_type out_s1, length_type rows, length_type cols)
   {
     plan_.out_of_place(reinterpret_cast<ovxhip::cscalar const *>(in), in_s0, in_s1,
                        reinterpret_cast<ovxhip::cscalar *>(out), out_s0, out_s1, rows, cols);
+  }
+  virtual void out_of_place(std::pair<float *, float *> in, stride_type in_s0, stride_type in_s1,
+                            std::pair<float *, float *> out, stride_type out_s0, stride_type out_s1, length_type rows,
+                            length_type cols)
+  {
+    plan_.out_of_place(ovxhip::const_split_ptr(in.first, in.second), in_s0, in_s1, out, out_s0, out_s1, rows, cols);
   }
   float scale() const { return scale_; }
 
@@ -78,20 +107,40 @@ private:
 template <int S>
 class fft : public signal::fft::fft_backend<1, ctype, ctype, S>
 {
+  typedef signal::fft::fft_backend<1, ctype, ctype, S> base_type;
   static int const dir = S == fft_fwd ? OVX_FFT_FWD : OVX_FFT_INV;
 
 public:
   fft(vsip::Domain<1> const &dom, float scale) : plan_(dom.size(), scale, OVX_FLAG_HOST_POINTERS) {}
   virtual char const *name() { return "ovxhip::fft"; }
   virtual bool supports_scale() { return true; }
+  virtual void query_layout(Rt_layout<1> &rtl_inout)
+  {
+    base_type::query_layout(rtl_inout);
+    rtl_inout.storage_format = vsip::Dense<1, ctype>::storage_format;
+  }
+  virtual void query_layout(Rt_layout<1> &rtl_in, Rt_layout<1> &rtl_out)
+  {
+    base_type::query_layout(rtl_in, rtl_out);
+    rtl_in.storage_format = rtl_out.storage_format = vsip::Dense<1, ctype>::storage_format;
+  }
   virtual void in_place(ctype *inout, stride_type s, length_type)
   {
     plan_.in_place(reinterpret_cast<ovxhip::cscalar *>(inout), s);
+  }
+  virtual void in_place(std::pair<float *, float *> inout, stride_type s, length_type)
+  {
+    plan_.in_place(inout, s);
   }
   virtual void out_of_place(ctype *in, stride_type in_s, ctype *out, stride_type out_s, length_type)
   {
     plan_.out_of_place(reinterpret_cast<ovxhip::cscalar const *>(in), in_s, reinterpret_cast<ovxhip::cscalar *>(out),
                        out_s);
+  }
+  virtual void out_of_place(std::pair<float *, float *> in, stride_type in_s, std::pair<float *, float *> out,
+                            stride_type out_s, length_type)
+  {
+    plan_.out_of_place(ovxhip::const_split_ptr(in.first, in.second), in_s, out, out_s);
   }
 
 private:
@@ -146,6 +195,21 @@ private:
 inline bool length_ok(length_type n) { return ovx_supported_length((long)n) != 0; }
 // a real transform of even length n runs on the complex kernels of n/2
 inline bool real_length_ok(length_type n) { return n % 2 == 0 && n >= 4 && length_ok(n / 2); }
+
+// The fused fastconv on what DDA hands over: interleaved complex pointers,
+// or (real, imaginary) pairs when the blocks use split storage.
+inline void fastconv_call(ovxhip::Fastconv &fc, ctype const *x, stride_type ldx, ctype const *w, ctype *y,
+                          stride_type ldy, length_type rows)
+{
+  fc(reinterpret_cast<ovxhip::cscalar const *>(x), ldx, reinterpret_cast<ovxhip::cscalar const *>(w), 0,
+     reinterpret_cast<ovxhip::cscalar *>(y), ldy, rows);
+}
+template <typename PX, typename PW>
+void fastconv_call(ovxhip::Fastconv &fc, std::pair<PX *, PX *> x, stride_type ldx, std::pair<PW *, PW *> w,
+                   std::pair<float *, float *> y, stride_type ldy, length_type rows)
+{
+  fc(ovxhip::const_split_ptr(x.first, x.second), ldx, ovxhip::const_split_ptr(w.first, w.second), 0, y, ldy, rows);
+}
 
 } // namespace ovxx::ovxhip_backend
 
@@ -267,9 +331,8 @@ struct Evaluator<op::assign<2>, be::user,
     dda::Data<MB, dda::in> m_data(m);
     dda::Data<LHS, dda::out> out_data(lhs);
     ovxhip::Fastconv fc(rows, cols, inv->scale(), OVX_FLAG_HOST_POINTERS);
-    fc(reinterpret_cast<ovxhip::cscalar const *>(m_data.ptr()), m_data.stride(0),
-       reinterpret_cast<ovxhip::cscalar const *>(w_data.ptr()), 0,
-       reinterpret_cast<ovxhip::cscalar *>(out_data.ptr()), out_data.stride(0), rows);
+    ovxhip_backend::fastconv_call(fc, m_data.ptr(), m_data.stride(0), w_data.ptr(), out_data.ptr(),
+                                  out_data.stride(0), rows);
   }
 };
 

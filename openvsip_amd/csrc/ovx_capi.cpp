@@ -749,6 +749,148 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
   return p->line.fastconv((const float2 *)x, ldx, (const float2 *)w, ldw, (float2 *)y, ldy, rows, p->scale, s);
 }
 
+// ---------------------------------------------------------------- split complex
+// Split-complex storage (OpenVSIP's split_complex format: real and imaginary
+// parts in two float arrays with the same strides), the fftm_backend
+// overloads on std::pair<T*, T*> (backend.hpp:741-783; the FFTW backend's
+// split DFTs, fftw.cpp:945-1001).  The lines are interleaved into the plan's
+// workspace, the interleaved path runs there in place, and the result is
+// split into the output arrays: two extra passes over the data, and the same
+// floats as the interleaved call on the same values.
+namespace {
+
+struct SplitIn {
+  const float *re, *im;
+};
+struct SplitOut {
+  float *re, *im;
+};
+
+// Host-pointer staging of one split side: both parts copied line by line into
+// one device buffer, the imaginary part `extent` bytes after the real part.
+hipError_t stage_split_in(void *dst, const void *re, const void *im, HostLines g, hipStream_t s)
+{
+  hipError_t e = copy_lines(dst, re, g, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = copy_lines((char *)dst + g.extent(), im, g, hipMemcpyHostToDevice, s);
+  return e;
+}
+
+SplitIn staged_in(void *buf, HostLines g) { return {(const float *)buf, (const float *)((char *)buf + g.extent())}; }
+SplitOut staged_out(void *buf, HostLines g) { return {(float *)buf, (float *)((char *)buf + g.extent())}; }
+
+} // namespace
+
+static int fftm_split_exec(ovx_plan p, const void *in_re, const void *in_im, long in_s0, long in_s1, void *out_re,
+                           void *out_im, long out_s0, long out_s1, long rows, long cols, void *stream)
+{
+  int st;
+  long n, lines, ldi, ldo;
+  if ((st = fftm_geometry(p, in_s0, in_s1, rows, cols, &n, &lines, &ldi))) return st;
+  if ((st = fftm_geometry(p, out_s0, out_s1, rows, cols, &n, &lines, &ldo))) return st;
+  if (n == 0 || lines == 0) return OVX_OK;
+  if (!in_re || !in_im || !out_re || !out_im) return fail(OVX_ERR_INVALID, "null data pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const int dir = sign_of(p->dir);
+  auto run = [&](SplitIn a, SplitOut b) -> int {
+    int r = p->split_ws.alloc((size_t)lines * n * 8);
+    if (r) return r;
+    float2 *ws = (float2 *)p->split_ws.p;
+    hipError_t e = launch_interleave(a.re, a.im, ldi, ws, n, lines, n, s);
+    if (e != hipSuccess) return hip_fail(e, "interleave");
+    if ((r = p->line.transform(dir, ws, n, ws, n, lines, p->scale, s))) return r;
+    e = launch_deinterleave(ws, n, b.re, b.im, ldo, lines, n, s);
+    return e == hipSuccess ? OVX_OK : hip_fail(e, "deinterleave");
+  };
+  if (!(p->flags & OVX_FLAG_HOST_POINTERS))
+    return run({(const float *)in_re, (const float *)in_im}, {(float *)out_re, (float *)out_im});
+  const HostLines gi = host_lines(lines, ldi, n, 4), go = host_lines(lines, ldo, n, 4);
+  if ((st = p->stage_in.alloc(2 * gi.extent())) || (st = p->stage_out.alloc(2 * go.extent()))) return st;
+  hipError_t e = stage_split_in(p->stage_in.p, in_re, in_im, gi, s);
+  if (e != hipSuccess) return hip_fail(e, "H2D copy");
+  if ((st = run(staged_in(p->stage_in.p, gi), staged_out(p->stage_out.p, go)))) return st;
+  SplitOut so = staged_out(p->stage_out.p, go);
+  e = copy_lines(out_re, so.re, go, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = copy_lines(out_im, so.im, go, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? OVX_OK : hip_fail(e, "D2H copy");
+}
+
+extern "C" int ovx_fftm_split_out_of_place(ovx_plan p, const void *in_re, const void *in_im, long in_s0, long in_s1,
+                                           void *out_re, void *out_im, long out_s0, long out_s1, long rows,
+                                           long cols, void *stream)
+{
+  int st = check_plan(p, PLAN_FFTM);
+  if (st) return st;
+  return fftm_split_exec(p, in_re, in_im, in_s0, in_s1, out_re, out_im, out_s0, out_s1, rows, cols, stream);
+}
+
+extern "C" int ovx_fftm_split_in_place(ovx_plan p, void *re, void *im, long s0, long s1, long rows, long cols,
+                                       void *stream)
+{
+  return ovx_fftm_split_out_of_place(p, re, im, s0, s1, re, im, s0, s1, rows, cols, stream);
+}
+
+extern "C" int ovx_fft_split_out_of_place(ovx_plan p, const void *in_re, const void *in_im, long in_stride,
+                                          void *out_re, void *out_im, long out_stride, long n, void *stream)
+{
+  int st = check_plan(p, PLAN_FFT);
+  if (st) return st;
+  if (n != p->cols) return fail(OVX_ERR_INVALID, "length %ld does not match the plan's %ld", n, p->cols);
+  if ((in_stride != 1 || out_stride != 1) && n > 1) return fail(OVX_ERR_UNSUPPORTED, "strides must be 1");
+  return fftm_split_exec(p, in_re, in_im, n, 1, out_re, out_im, n, 1, 1, n, stream);
+}
+
+extern "C" int ovx_fft_split_in_place(ovx_plan p, void *re, void *im, long stride, long n, void *stream)
+{
+  return ovx_fft_split_out_of_place(p, re, im, stride, re, im, stride, n, stream);
+}
+
+extern "C" int ovx_fastconv_split_exec(ovx_plan p, const void *x_re, const void *x_im, long ldx, const void *w_re,
+                                       const void *w_im, long ldw, void *y_re, void *y_im, long ldy, long rows,
+                                       void *stream)
+{
+  int st = check_plan(p, PLAN_FASTCONV);
+  if (st) return st;
+  if (rows < 0 || rows > p->rows) return fail(OVX_ERR_INVALID, "rows %ld outside the plan's %ld", rows, p->rows);
+  const long n = p->cols;
+  if (rows > 1 && (ldx < n || ldy < n)) return fail(OVX_ERR_INVALID, "row stride shorter than n");
+  if (ldw != 0 && rows > 1 && ldw < n) return fail(OVX_ERR_INVALID, "coefficient row stride shorter than n");
+  if (rows == 0) return OVX_OK;
+  if (!x_re || !x_im || !w_re || !w_im || !y_re || !y_im) return fail(OVX_ERR_INVALID, "null data pointer");
+  if (p->bf16x3) return fail(OVX_ERR_UNSUPPORTED, "the BF16x3 variant takes interleaved data");
+  hipStream_t s = (hipStream_t)stream;
+  const long wlines = ldw ? rows : 1;
+  auto run = [&](SplitIn x, SplitIn w, SplitOut y) -> int {
+    int r = p->split_ws.alloc((size_t)(rows + wlines) * n * 8);
+    if (r) return r;
+    float2 *wx = (float2 *)p->split_ws.p, *ww = wx + (size_t)rows * n;
+    hipError_t e = launch_interleave(x.re, x.im, ldx, wx, n, rows, n, s);
+    if (e == hipSuccess) e = launch_interleave(w.re, w.im, ldw, ww, n, wlines, n, s);
+    if (e != hipSuccess) return hip_fail(e, "interleave");
+    if ((r = p->line.fastconv(wx, n, ww, ldw ? n : 0, wx, n, rows, p->scale, s))) return r;
+    e = launch_deinterleave(wx, n, y.re, y.im, ldy, rows, n, s);
+    return e == hipSuccess ? OVX_OK : hip_fail(e, "deinterleave");
+  };
+  if (!(p->flags & OVX_FLAG_HOST_POINTERS))
+    return run({(const float *)x_re, (const float *)x_im}, {(const float *)w_re, (const float *)w_im},
+               {(float *)y_re, (float *)y_im});
+  const HostLines gx = host_lines(rows, ldx, n, 4), gw = host_lines(wlines, ldw, n, 4),
+                  gy = host_lines(rows, ldy, n, 4);
+  if ((st = p->stage_in.alloc(2 * gx.extent())) || (st = p->stage_w.alloc(2 * gw.extent())) ||
+      (st = p->stage_out.alloc(2 * gy.extent())))
+    return st;
+  hipError_t e = stage_split_in(p->stage_in.p, x_re, x_im, gx, s);
+  if (e == hipSuccess) e = stage_split_in(p->stage_w.p, w_re, w_im, gw, s);
+  if (e != hipSuccess) return hip_fail(e, "H2D copy");
+  if ((st = run(staged_in(p->stage_in.p, gx), staged_in(p->stage_w.p, gw), staged_out(p->stage_out.p, gy))))
+    return st;
+  SplitOut so = staged_out(p->stage_out.p, gy);
+  e = copy_lines(y_re, so.re, gy, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = copy_lines(y_im, so.im, gy, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? OVX_OK : hip_fail(e, "D2H copy");
+}
+
 // ---------------------------------------------------------------- replicas
 // A replica spectrum kept resident on the device, in natural order and, for
 // the default kernel path of its length, prepared once in the layout that

@@ -121,7 +121,8 @@ const char *ovx_plan_s::name() const
 
 size_t ovx_plan_s::device_bytes() const
 {
-  size_t b = line.device_bytes() + stage_in.bytes + stage_out.bytes + stage_w.bytes + u256.bytes + wtab.bytes;
+  size_t b = line.device_bytes() + stage_in.bytes + stage_out.bytes + stage_w.bytes + u256.bytes + wtab.bytes +
+             split_ws.bytes;
   if (corr)
     b += corr->fftL.device_bytes() + corr->hpad.bytes + corr->H.bytes + corr->ref_stage.bytes +
          corr->in_stage.bytes + corr->out_stage.bytes;

@@ -50,4 +50,11 @@ hipError_t launch_scale_copy(const float2 *in, float2 *out, long n, float scale,
 // Table of exp(-2 pi i j / n), j in [0, count), FP64-evaluated, FP32-rounded.
 hipError_t launch_unit_roots(long n, long count, float2 *out, hipStream_t s);
 
+// Split-complex <-> interleaved, `lines` lines of n values: line l of the
+// float arrays re/im starts at l*ld_s, of the complex array at l*ld_c.
+hipError_t launch_interleave(const float *re, const float *im, long ld_s, float2 *out, long ld_c, long lines,
+                             long n, hipStream_t s);
+hipError_t launch_deinterleave(const float2 *in, long ld_c, float *re, float *im, long ld_s, long lines, long n,
+                               hipStream_t s);
+
 } // namespace ovx

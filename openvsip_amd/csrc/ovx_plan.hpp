@@ -85,6 +85,7 @@ struct ovx_plan_s {
   bool bf16x3 = false; // fastconv radix-16 stages on MFMA (precision variant)
   ovx::DevBuf u256;    // its 256th roots of unity
   ovx::DevBuf wtab;    // real Fftm: exp(-2 pi i k / n), k <= n/4
+  ovx::DevBuf split_ws; // split-complex calls: interleaved copy of the lines
 
   ~ovx_plan_s();
   const char *name() const;

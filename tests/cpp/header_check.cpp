@@ -9,6 +9,19 @@
 
 static int fails = 0;
 
+// compiled, not run (plans need a device): the split-storage overloads
+void split_overloads(ovxhip::Fftm<OVX_ROW, OVX_FFT_FWD> &f, ovxhip::Fft<OVX_FFT_INV> &g, ovxhip::Fastconv &c,
+                     float *re, float *im)
+{
+  ovxhip::split_ptr z(re, im);
+  ovxhip::const_split_ptr cz(re, im);
+  f.out_of_place(cz, 16, 1, z, 16, 1, 4, 16);
+  f.in_place(z, 16, 1, 4, 16);
+  g.out_of_place(cz, 1, z, 1);
+  g.in_place(z, 1);
+  c(cz, 16, cz, 0, z, 16, 4);
+}
+
 #define EXPECT_THROW(stmt, type)                                  \
   do {                                                            \
     try {                                                         \
