@@ -213,6 +213,16 @@ int ovx_fft_split_out_of_place(ovx_plan plan, const void *in_re, const void *in_
                                long out_stride, long n, void *stream);
 int ovx_fft_split_in_place(ovx_plan plan, void *re, void *im, long stride, long n,
                            void *stream);
+/* Real Fftm with a split complex side (backend.hpp:600-694 split overloads;
+   FFTW split r2c / c2r, fftw.cpp:778-903): r2c on a forward plan (real in,
+   split out), c2r on an inverse plan (split in, real out); strides as in
+   ovx_fftm_real_out_of_place, the complex side's in floats of each part. */
+int ovx_fftm_r2c_split(ovx_plan plan, const void *in, long in_s0, long in_s1,
+                       void *out_re, void *out_im, long out_s0, long out_s1,
+                       long rows, long cols, void *stream);
+int ovx_fftm_c2r_split(ovx_plan plan, const void *in_re, const void *in_im,
+                       long in_s0, long in_s1, void *out, long out_s0,
+                       long out_s1, long rows, long cols, void *stream);
 /* ovx_fastconv_exec on split X, W and Y (W one vector when ldw == 0).  Not
    for OVX_FLAG_BF16X3 plans. */
 int ovx_fastconv_split_exec(ovx_plan plan, const void *x_re, const void *x_im,

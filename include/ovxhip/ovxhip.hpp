@@ -135,6 +135,17 @@ public:
   {
     check(ovx_fftm_real_out_of_place(p_.get(), in, in_s0, in_s1, out, out_s0, out_s1, rows, cols, stream));
   }
+  // split complex side: r2c (D == OVX_FFT_FWD) and c2r (OVX_FFT_INV)
+  void out_of_place(float const *in, long in_s0, long in_s1, split_ptr out, long out_s0, long out_s1, long rows,
+                    long cols, void *stream = nullptr)
+  {
+    check(ovx_fftm_r2c_split(p_.get(), in, in_s0, in_s1, out.first, out.second, out_s0, out_s1, rows, cols, stream));
+  }
+  void out_of_place(const_split_ptr in, long in_s0, long in_s1, float *out, long out_s0, long out_s1, long rows,
+                    long cols, void *stream = nullptr)
+  {
+    check(ovx_fftm_c2r_split(p_.get(), in.first, in.second, in_s0, in_s1, out, out_s0, out_s1, rows, cols, stream));
+  }
   char const *name() const { return p_.name(); }
 
 private:

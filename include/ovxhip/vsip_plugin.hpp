@@ -17,10 +17,10 @@
 // through device buffers); an application that keeps its data on the device
 // calls include/ovxhip.h directly.
 //
-// The complex Fft/Fftm backends ask DDA for the storage format of OpenVSIP's
-// dense blocks (interleaved, or split when OpenVSIP is configured with split
-// complex storage), so neither format is converted on the host; split data
-// take the library's split entry points.
+// The backends ask DDA for the storage format of OpenVSIP's dense blocks on
+// their complex sides (interleaved, or split when OpenVSIP is configured
+// with split complex storage), so neither format is converted on the host;
+// split data take the library's split entry points.
 //
 // This header compiles only against an OpenVSIP tree; it is not built by this
 // repository's tests.
@@ -152,6 +152,7 @@ private:
 template <int A>
 class fftm_r2c : public signal::fft::fftm_backend<float, ctype, A, fft_fwd>
 {
+  typedef signal::fft::fftm_backend<float, ctype, A, fft_fwd> base_type;
   static int const axis = A == vsip::col ? OVX_COL : OVX_ROW;
 
 public:
@@ -160,10 +161,21 @@ public:
   {}
   virtual char const *name() { return "ovxhip::fftm_r2c"; }
   virtual bool supports_scale() { return true; }
+  // the complex side in the dense blocks' storage format (no DDA conversion)
+  virtual void query_layout(Rt_layout<2> &rtl_in, Rt_layout<2> &rtl_out)
+  {
+    base_type::query_layout(rtl_in, rtl_out);
+    rtl_out.storage_format = vsip::Dense<1, ctype>::storage_format;
+  }
   virtual void out_of_place(float *in, stride_type in_s0, stride_type in_s1, ctype *out, stride_type out_s0,
                             stride_type out_s1, length_type rows, length_type cols)
   {
     plan_.out_of_place(in, in_s0, in_s1, out, out_s0, out_s1, rows, cols);
+  }
+  virtual void out_of_place(float *in, stride_type in_s0, stride_type in_s1, std::pair<float *, float *> out,
+                            stride_type out_s0, stride_type out_s1, length_type rows, length_type cols)
+  {
+    plan_.out_of_place(static_cast<float const *>(in), in_s0, in_s1, out, out_s0, out_s1, rows, cols);
   }
 
 private:
@@ -174,6 +186,7 @@ private:
 template <int A>
 class fftm_c2r : public signal::fft::fftm_backend<ctype, float, A, fft_inv>
 {
+  typedef signal::fft::fftm_backend<ctype, float, A, fft_inv> base_type;
   static int const axis = A == vsip::col ? OVX_COL : OVX_ROW;
 
 public:
@@ -182,10 +195,20 @@ public:
   {}
   virtual char const *name() { return "ovxhip::fftm_c2r"; }
   virtual bool supports_scale() { return true; }
+  virtual void query_layout(Rt_layout<2> &rtl_in, Rt_layout<2> &rtl_out)
+  {
+    base_type::query_layout(rtl_in, rtl_out);
+    rtl_in.storage_format = vsip::Dense<1, ctype>::storage_format;
+  }
   virtual void out_of_place(ctype *in, stride_type in_s0, stride_type in_s1, float *out, stride_type out_s0,
                             stride_type out_s1, length_type rows, length_type cols)
   {
     plan_.out_of_place(in, in_s0, in_s1, out, out_s0, out_s1, rows, cols);
+  }
+  virtual void out_of_place(std::pair<float *, float *> in, stride_type in_s0, stride_type in_s1, float *out,
+                            stride_type out_s0, stride_type out_s1, length_type rows, length_type cols)
+  {
+    plan_.out_of_place(ovxhip::const_split_ptr(in.first, in.second), in_s0, in_s1, out, out_s0, out_s1, rows, cols);
   }
 
 private:

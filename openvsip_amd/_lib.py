@@ -74,6 +74,8 @@ SIGNATURES = {
     "ovx_fft_split_out_of_place": (_I, [_P, _P, _P, _L, _P, _P, _L, _L, _P]),
     "ovx_fft_split_in_place": (_I, [_P, _P, _P, _L, _L, _P]),
     "ovx_fastconv_split_exec": (_I, [_P, _P, _P, _L, _P, _P, _L, _P, _P, _L, _L, _P]),
+    "ovx_fftm_r2c_split": (_I, [_P, _P, _L, _L, _P, _P, _L, _L, _L, _L, _P]),
+    "ovx_fftm_c2r_split": (_I, [_P, _P, _P, _L, _L, _P, _L, _L, _L, _L, _P]),
     "ovx_storage_create": (_I, [_PP, ctypes.c_size_t, _P]),
     "ovx_storage_destroy": (_I, [_P]),
     "ovx_storage_ptr": (_I, [_P, _I, _I, _PP, _P]),

@@ -638,43 +638,70 @@ static int real_side(ovx_plan p, long s0, long s1, long len, long *lines, long *
   return OVX_OK;
 }
 
+namespace {
+// Lines of a real Fftm call: n reals per line on the real side (line stride
+// ldr floats, read as ldz complex pairs), h + 1 values on the complex side.
+struct RealLines {
+  bool fwd;
+  long n, h, lines, ldr, ldc, ldz;
+};
+} // namespace
+
+// Geometry checks shared by the interleaved and split real entry points;
+// `real_ptr` is the real side's data (8-byte aligned: read as pairs).
+static int real_lines(ovx_plan p, long in_s0, long in_s1, long out_s0, long out_s1, long rows, long cols,
+                      RealLines *g)
+{
+  if (rows != p->rows || cols != p->cols)
+    return fail(OVX_ERR_INVALID, "size %ldx%ld does not match the plan's %ldx%ld", rows, cols, p->rows, p->cols);
+  g->fwd = p->dir == OVX_FFT_FWD;
+  g->n = p->axis == OVX_ROW ? cols : rows;
+  g->h = g->n / 2;
+  int st;
+  // real side: n floats per line; complex side: h + 1 values per line
+  if ((st = real_side(p, g->fwd ? in_s0 : out_s0, g->fwd ? in_s1 : out_s1, g->n, &g->lines, &g->ldr))) return st;
+  if ((st = real_side(p, g->fwd ? out_s0 : in_s0, g->fwd ? out_s1 : in_s1, g->h + 1, &g->lines, &g->ldc))) return st;
+  if (g->n > 0 && g->lines > 1 && (g->ldr % 2))
+    return fail(OVX_ERR_UNSUPPORTED, "real line stride %ld must be even (lines are read as complex pairs)", g->ldr);
+  g->ldz = g->lines > 1 ? g->ldr / 2 : g->h; // real line as h complex values
+  if (g->n && g->lines && (!p->line.tables_ready() || !p->wtab.p))
+    return fail(OVX_ERR_STATE, "real Fftm tables missing");
+  return OVX_OK;
+}
+
+// The device body: forward = h-point complex transform of the real line read
+// as pairs, then the r2c post-pass; inverse = c2r pre-pass, then the inverse
+// h-point transform in place.  `ldc` is the complex side's line stride.
+static int real_run(ovx_plan p, const RealLines &g, const void *din, void *dout, long ldc, hipStream_t s)
+{
+  if (g.fwd) {
+    int r = p->line.transform(-1, (const float2 *)din, g.ldz, (float2 *)dout, ldc, g.lines, 1.0f, s);
+    if (r) return r;
+    hipError_t e = launch_r2c_post((float2 *)dout, ldc, g.lines, g.h, (const float2 *)p->wtab.p, p->scale, s);
+    return e == hipSuccess ? OVX_OK : hip_fail(e, "r2c pass");
+  }
+  hipError_t e = launch_c2r_pre((const float2 *)din, ldc, (float2 *)dout, g.ldz, g.lines, g.h,
+                                (const float2 *)p->wtab.p, s);
+  if (e != hipSuccess) return hip_fail(e, "c2r pass");
+  return p->line.transform(+1, (const float2 *)dout, g.ldz, (float2 *)dout, g.ldz, g.lines, p->scale, s);
+}
+
 extern "C" int ovx_fftm_real_out_of_place(ovx_plan p, const void *in, long in_s0, long in_s1, void *out, long out_s0,
                                           long out_s1, long rows, long cols, void *stream)
 {
   int st = check_plan(p, PLAN_FFTM_REAL);
   if (st) return st;
-  if (rows != p->rows || cols != p->cols)
-    return fail(OVX_ERR_INVALID, "size %ldx%ld does not match the plan's %ldx%ld", rows, cols, p->rows, p->cols);
-  const bool fwd = p->dir == OVX_FFT_FWD;
-  const long n = p->axis == OVX_ROW ? cols : rows, h = n / 2;
-  long lines, ldr, ldc;
-  // real side: n floats per line; complex side: h + 1 values per line
-  if ((st = real_side(p, fwd ? in_s0 : out_s0, fwd ? in_s1 : out_s1, n, &lines, &ldr))) return st;
-  if ((st = real_side(p, fwd ? out_s0 : in_s0, fwd ? out_s1 : in_s1, h + 1, &lines, &ldc))) return st;
-  if (n == 0 || lines == 0) return OVX_OK; // empty views may carry null pointers
+  RealLines g;
+  if ((st = real_lines(p, in_s0, in_s1, out_s0, out_s1, rows, cols, &g))) return st;
+  if (g.n == 0 || g.lines == 0) return OVX_OK; // empty views may carry null pointers
   if (!in || !out) return fail(OVX_ERR_INVALID, "null data pointer");
   if (in == out) return fail(OVX_ERR_UNSUPPORTED, "real Fftm runs out of place (input and output differ in type)");
-  if (lines > 1 && (ldr % 2))
-    return fail(OVX_ERR_UNSUPPORTED, "real line stride %ld must be even (lines are read as complex pairs)", ldr);
-  const void *real_ptr = fwd ? in : out;
-  if (((uintptr_t)real_ptr) % 8) return fail(OVX_ERR_UNSUPPORTED, "real data must be 8-byte aligned");
-  if (!p->line.tables_ready() || !p->wtab.p) return fail(OVX_ERR_STATE, "real Fftm tables missing");
+  if (((uintptr_t)(g.fwd ? in : out)) % 8) return fail(OVX_ERR_UNSUPPORTED, "real data must be 8-byte aligned");
   hipStream_t s = (hipStream_t)stream;
-  const long ldz = lines > 1 ? ldr / 2 : h; // real line as h complex values
-  auto run = [&](const void *din, void *dout) -> int {
-    if (fwd) {
-      int r = p->line.transform(-1, (const float2 *)din, ldz, (float2 *)dout, ldc, lines, 1.0f, s);
-      if (r) return r;
-      hipError_t e = launch_r2c_post((float2 *)dout, ldc, lines, h, (const float2 *)p->wtab.p, p->scale, s);
-      return e == hipSuccess ? OVX_OK : hip_fail(e, "r2c pass");
-    }
-    hipError_t e = launch_c2r_pre((const float2 *)din, ldc, (float2 *)dout, ldz, lines, h, (const float2 *)p->wtab.p, s);
-    if (e != hipSuccess) return hip_fail(e, "c2r pass");
-    return p->line.transform(+1, (const float2 *)dout, ldz, (float2 *)dout, ldz, lines, p->scale, s);
-  };
+  auto run = [&](const void *din, void *dout) -> int { return real_run(p, g, din, dout, g.ldc, s); };
   if (p->flags & OVX_FLAG_HOST_POINTERS) {
-    const HostLines gr = host_lines(lines, ldr, n, 4), gc = host_lines(lines, ldc, h + 1, 8);
-    return with_staging(p, in, fwd ? gr : gc, out, fwd ? gc : gr, s, run);
+    const HostLines gr = host_lines(g.lines, g.ldr, g.n, 4), gc = host_lines(g.lines, g.ldc, g.h + 1, 8);
+    return with_staging(p, in, g.fwd ? gr : gc, out, g.fwd ? gc : gr, s, run);
   }
   return run(in, out);
 }
@@ -887,6 +914,78 @@ extern "C" int ovx_fastconv_split_exec(ovx_plan p, const void *x_re, const void 
   SplitOut so = staged_out(p->stage_out.p, gy);
   e = copy_lines(y_re, so.re, gy, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = copy_lines(y_im, so.im, gy, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? OVX_OK : hip_fail(e, "D2H copy");
+}
+
+// Real Fftm with a split complex side (fftm_backend<T, complex<T>> and
+// <complex<T>, T> split overloads, backend.hpp:600-694; FFTW's split r2c /
+// c2r, fftw.cpp:778-903): the complex side goes through the plan's
+// interleaved workspace (h + 1 values per line, dense).
+extern "C" int ovx_fftm_r2c_split(ovx_plan p, const void *in, long in_s0, long in_s1, void *out_re, void *out_im,
+                                  long out_s0, long out_s1, long rows, long cols, void *stream)
+{
+  int st = check_plan(p, PLAN_FFTM_REAL);
+  if (st) return st;
+  if (p->dir != OVX_FFT_FWD) return fail(OVX_ERR_INVALID, "an inverse real Fftm plan runs complex -> real");
+  RealLines g;
+  if ((st = real_lines(p, in_s0, in_s1, out_s0, out_s1, rows, cols, &g))) return st;
+  if (g.n == 0 || g.lines == 0) return OVX_OK;
+  if (!in || !out_re || !out_im) return fail(OVX_ERR_INVALID, "null data pointer");
+  const bool host = p->flags & OVX_FLAG_HOST_POINTERS;
+  if (!host && ((uintptr_t)in) % 8) return fail(OVX_ERR_UNSUPPORTED, "real data must be 8-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const long hc = g.h + 1;
+  auto run = [&](const void *din, SplitOut o) -> int {
+    int r = p->split_ws.alloc((size_t)g.lines * hc * 8);
+    if (r) return r;
+    float2 *ws = (float2 *)p->split_ws.p;
+    if ((r = real_run(p, g, din, ws, hc, s))) return r;
+    hipError_t e = launch_deinterleave(ws, hc, o.re, o.im, g.ldc, g.lines, hc, s);
+    return e == hipSuccess ? OVX_OK : hip_fail(e, "deinterleave");
+  };
+  if (!host) return run(in, {(float *)out_re, (float *)out_im});
+  const HostLines gr = host_lines(g.lines, g.ldr, g.n, 4), gc = host_lines(g.lines, g.ldc, hc, 4);
+  if ((st = p->stage_in.alloc(gr.extent())) || (st = p->stage_out.alloc(2 * gc.extent()))) return st;
+  hipError_t e = copy_lines(p->stage_in.p, in, gr, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "H2D copy");
+  if ((st = run(p->stage_in.p, staged_out(p->stage_out.p, gc)))) return st;
+  SplitOut so = staged_out(p->stage_out.p, gc);
+  e = copy_lines(out_re, so.re, gc, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = copy_lines(out_im, so.im, gc, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? OVX_OK : hip_fail(e, "D2H copy");
+}
+
+extern "C" int ovx_fftm_c2r_split(ovx_plan p, const void *in_re, const void *in_im, long in_s0, long in_s1, void *out,
+                                  long out_s0, long out_s1, long rows, long cols, void *stream)
+{
+  int st = check_plan(p, PLAN_FFTM_REAL);
+  if (st) return st;
+  if (p->dir != OVX_FFT_INV) return fail(OVX_ERR_INVALID, "a forward real Fftm plan runs real -> complex");
+  RealLines g;
+  if ((st = real_lines(p, in_s0, in_s1, out_s0, out_s1, rows, cols, &g))) return st;
+  if (g.n == 0 || g.lines == 0) return OVX_OK;
+  if (!in_re || !in_im || !out) return fail(OVX_ERR_INVALID, "null data pointer");
+  const bool host = p->flags & OVX_FLAG_HOST_POINTERS;
+  if (!host && ((uintptr_t)out) % 8) return fail(OVX_ERR_UNSUPPORTED, "real data must be 8-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const long hc = g.h + 1;
+  auto run = [&](SplitIn i, void *dout) -> int {
+    int r = p->split_ws.alloc((size_t)g.lines * hc * 8);
+    if (r) return r;
+    float2 *ws = (float2 *)p->split_ws.p;
+    hipError_t e = launch_interleave(i.re, i.im, g.ldc, ws, hc, g.lines, hc, s);
+    if (e != hipSuccess) return hip_fail(e, "interleave");
+    return real_run(p, g, ws, dout, hc, s);
+  };
+  if (!host) return run({(const float *)in_re, (const float *)in_im}, out);
+  const HostLines gr = host_lines(g.lines, g.ldr, g.n, 4), gc = host_lines(g.lines, g.ldc, hc, 4);
+  if ((st = p->stage_in.alloc(2 * gc.extent())) || (st = p->stage_out.alloc(gr.extent()))) return st;
+  hipError_t e = stage_split_in(p->stage_in.p, in_re, in_im, gc, s);
+  if (e != hipSuccess) return hip_fail(e, "H2D copy");
+  if ((st = run(staged_in(p->stage_in.p, gc), p->stage_out.p))) return st;
+  e = copy_lines(out, p->stage_out.p, gr, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e == hipSuccess ? OVX_OK : hip_fail(e, "D2H copy");
 }

@@ -22,6 +22,13 @@ void split_overloads(ovxhip::Fftm<OVX_ROW, OVX_FFT_FWD> &f, ovxhip::Fft<OVX_FFT_
   c(cz, 16, cz, 0, z, 16, 4);
 }
 
+void real_split_overloads(ovxhip::FftmReal<OVX_ROW, OVX_FFT_FWD> &r, ovxhip::FftmReal<OVX_ROW, OVX_FFT_INV> &c,
+                          float *x, float *re, float *im)
+{
+  r.out_of_place(x, 16, 1, ovxhip::split_ptr(re, im), 9, 1, 4, 16);
+  c.out_of_place(ovxhip::const_split_ptr(re, im), 9, 1, x, 16, 1, 4, 16);
+}
+
 #define EXPECT_THROW(stmt, type)                                  \
   do {                                                            \
     try {                                                         \

@@ -243,3 +243,77 @@ def test_split_refusals(ovx):
         assert lib.ovx_fft_split_in_place(f, P(t), P(t), 1, 64, None) == _lib.OVX_ERR_INVALID
     finally:
         lib.ovx_plan_destroy(f)
+
+
+def plan_real(rows, n, direction, scale, flags=0):
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().ovx_fftm_real_plan_create(ctypes.byref(h), rows, n, _lib.ROW, direction, scale, flags))
+    return h
+
+
+# real lengths through the register-Stockham, hierarchical and four-step
+# paths (half lengths 8 ... 65536); 128 and 16 are tests/fft.cpp:256-257
+@pytest.mark.parametrize("rows,n", [(1, 128), (3, 16), (5, 2048), (4, 32768), (2, 131072)])
+def test_real_fftm_split_equals_interleaved(ovx, rows, n):
+    """r2c into split arrays and c2r from them, with padded complex strides,
+    against the interleaved real Fftm of the same plans: equal bit for bit;
+    the round trip returns the input."""
+    import torch
+    lib = _lib.lib()
+    h = n // 2 + 1
+    ldc = h + 3
+    rng = np.random.default_rng(n + rows)
+    x = rng.standard_normal((rows, n)).astype(np.float32)
+    fwd = plan_real(rows, n, _lib.FFT_FWD, 1.0)
+    inv = plan_real(rows, n, _lib.FFT_INV, 1.0 / n)
+    try:
+        xd = torch.from_numpy(x).cuda()
+        sr = torch.full((rows, ldc), 2.0, device="cuda")
+        si = torch.full((rows, ldc), -2.0, device="cuda")
+        _lib.check(lib.ovx_fftm_r2c_split(fwd, P(xd), n, 1, P(sr), P(si), ldc, 1, rows, n, None))
+        zc = torch.zeros((rows, ldc), dtype=torch.complex64, device="cuda")
+        _lib.check(lib.ovx_fftm_real_out_of_place(fwd, P(xd), n, 1, P(zc), ldc, 1, rows, n, None))
+        spec = as_c64(sr, si)
+        assert np.array_equal(spec[:, :h], host(zc)[:, :h])
+        assert np.all(host(sr)[:, h:] == 2.0) and np.all(host(si)[:, h:] == -2.0)
+        assert o.rel_l2(spec[:, :h], o.rfftm64(x)) < 1e-5
+        back = torch.zeros((rows, n + 2), device="cuda")
+        _lib.check(lib.ovx_fftm_c2r_split(inv, P(sr), P(si), ldc, 1, P(back), n + 2, 1, rows, n, None))
+        ref_back = torch.zeros((rows, n + 2), device="cuda")
+        _lib.check(lib.ovx_fftm_real_out_of_place(inv, P(zc), ldc, 1, P(ref_back), n + 2, 1, rows, n, None))
+        assert np.array_equal(host(back), host(ref_back))
+        assert o.rel_l2(host(back)[:, :n].astype(np.complex128), x.astype(np.complex128)) < 1e-5
+        # the directions are the plans'
+        assert lib.ovx_fftm_r2c_split(inv, P(xd), n, 1, P(sr), P(si), ldc, 1, rows, n,
+                                      None) == _lib.OVX_ERR_INVALID
+        assert lib.ovx_fftm_c2r_split(fwd, P(sr), P(si), ldc, 1, P(back), n + 2, 1, rows, n,
+                                      None) == _lib.OVX_ERR_INVALID
+    finally:
+        lib.ovx_plan_destroy(fwd)
+        lib.ovx_plan_destroy(inv)
+
+
+def test_real_fftm_split_host_pointers(ovx):
+    lib = _lib.lib()
+    rows, n = 3, 4096
+    h = n // 2 + 1
+    rng = np.random.default_rng(5)
+    x = np.full((rows, n + 4), 6.0, np.float32)
+    x[:, :n] = rng.standard_normal((rows, n))
+    fwd = plan_real(rows, n, _lib.FFT_FWD, 1.0, HOST)
+    inv = plan_real(rows, n, _lib.FFT_INV, 1.0 / n, HOST)
+    try:
+        sr = np.full((rows, h + 1), 8.0, np.float32)
+        si = np.full((rows, h + 1), 8.0, np.float32)
+        _lib.check(lib.ovx_fftm_r2c_split(fwd, x.ctypes.data, n + 4, 1, sr.ctypes.data, si.ctypes.data, h + 1, 1,
+                                          rows, n, None))
+        assert o.rel_l2((sr + 1j * si)[:, :h], o.rfftm64(x[:, :n])) < 1e-5
+        assert np.all(sr[:, h:] == 8.0) and np.all(si[:, h:] == 8.0)
+        back = np.full((rows, n + 2), -3.0, np.float32)
+        _lib.check(lib.ovx_fftm_c2r_split(inv, sr.ctypes.data, si.ctypes.data, h + 1, 1, back.ctypes.data, n + 2, 1,
+                                          rows, n, None))
+        assert o.rel_l2(back[:, :n].astype(np.complex128), x[:, :n].astype(np.complex128)) < 1e-5
+        assert np.all(back[:, n:] == -3.0)
+    finally:
+        lib.ovx_plan_destroy(fwd)
+        lib.ovx_plan_destroy(inv)
