@@ -15,10 +15,14 @@
 #define OVXHIP_HPP
 
 #include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
 #include <new>
-#include <utility>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <utility>
 
 #include "../ovxhip.h"
 
@@ -212,6 +216,22 @@ private:
   long rows_, n_;
   plan_holder p_;
 };
+
+// The Fastconv plan for (rows, n, scale, flags), made on first use and kept
+// for the calling thread (a plan is not safe for concurrent calls), so that
+// an evaluator that sees one expression shape per call -- the VSIPL++ fused
+// fastconv -- does not rebuild twiddle tables and workspaces every time.
+// The plans live until the thread ends.
+inline Fastconv &cached_fastconv(long rows, long n, float scale, unsigned flags = 0)
+{
+  typedef std::tuple<long, long, uint32_t, unsigned> key_type;
+  thread_local std::map<key_type, std::unique_ptr<Fastconv> > plans;
+  uint32_t bits;
+  std::memcpy(&bits, &scale, sizeof bits);
+  std::unique_ptr<Fastconv> &slot = plans[key_type(rows, n, bits, flags)];
+  if (!slot) slot.reset(new Fastconv(rows, n, scale, flags));
+  return *slot;
+}
 
 // vsip::Correlation<const_Vector, R, complex<float>> (corr.hpp:55-122).
 class Correlation {

@@ -353,7 +353,7 @@ struct Evaluator<op::assign<2>, be::user,
     dda::Data<VB, dda::in> w_data(w);
     dda::Data<MB, dda::in> m_data(m);
     dda::Data<LHS, dda::out> out_data(lhs);
-    ovxhip::Fastconv fc(rows, cols, inv->scale(), OVX_FLAG_HOST_POINTERS);
+    ovxhip::Fastconv &fc = ovxhip::cached_fastconv(rows, cols, inv->scale(), OVX_FLAG_HOST_POINTERS);
     ovxhip_backend::fastconv_call(fc, m_data.ptr(), m_data.stride(0), w_data.ptr(), out_data.ptr(),
                                   out_data.stride(0), rows);
   }

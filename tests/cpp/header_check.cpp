@@ -60,6 +60,9 @@ int main()
     if (e.code() != OVX_ERR_UNSUPPORTED) ++fails;
   }
   if (ovx_plan_destroy(nullptr) != OVX_ERR_INVALID) ++fails;
+  // a plan the cache could not make is not cached: the error repeats
+  EXPECT_THROW(ovxhip::cached_fastconv(4, 4096, 1.0f, OVX_FLAG_BF16X3), ovxhip::unimplemented);
+  EXPECT_THROW(ovxhip::cached_fastconv(4, 4096, 1.0f, OVX_FLAG_BF16X3), ovxhip::unimplemented);
   // user storage on the host needs no device: the application's array is
   // the host location, valid, and released / admitted without a copy
   {

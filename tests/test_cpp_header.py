@@ -22,3 +22,21 @@ def test_cpp_header_compiles_links_and_maps_errors(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("OK")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_cpp_host_api_on_gpu(tmp_path):
+    """The C++ header end to end on the device, as the VSIPL++ plugin uses
+    it: host pointers, split storage, the fastconv plan cache and Storage
+    (tests/cpp/host_api_gpu.cpp, checked against an FP64 DFT)."""
+    if not os.path.exists(os.path.join(LIB_DIR, "libovxhip.so")):
+        pytest.fail("libovxhip.so not built")
+    exe = str(tmp_path / "host_api_gpu")
+    src = os.path.join(ROOT, "tests", "cpp", "host_api_gpu.cpp")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src,
+                    "-L", LIB_DIR, "-lovxhip", "-Wl,-rpath," + LIB_DIR, "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("OK")
