@@ -267,40 +267,60 @@ int LineFFT::transform_ws(int dir, const float2 *in, long ld_in, float2 *out, lo
   }
 }
 
+// The hierarchical fastconv launch (interleaved or split X and Y, set in
+// `a`): a single spectrum is permuted once into the kernel's [k1][k2] order
+// with the scale folded in; a coefficient matrix is read in natural order.
+int LineFFT::hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *w, long ldw, long ldy, long lines,
+                           float scale, hipStream_t s) const
+{
+  if (!offsets_fit(hier_rows_per_block(n), std::max(std::max(ldx, ldy), ldw)))
+    return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
+  a.ldx = ldx;
+  a.ldy = ldy;
+  a.rows = lines;
+  if (ldw == 0) {
+    int st = wp.alloc((size_t)n * 8);
+    if (st) return st;
+    hipError_t e = launch_hier_permute(w, 0, (float2 *)wp.p, n, 1, scale, s);
+    if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
+    a.w = (const float2 *)wp.p;
+    a.ldw = 0;
+    a.natural = 0;
+    a.scale = 1.0f;
+  } else {
+    a.w = w;
+    a.ldw = ldw;
+    a.natural = 1;
+    a.scale = scale;
+  }
+  hipError_t e = launch_hier(launch_kind, n, a, s);
+  return e == hipSuccess ? OVX_OK : hip_fail(e, "fastconv launch");
+}
+
+int LineFFT::fastconv_split(const float *xr, const float *xi, long ldx, const float2 *w, long ldw, float *yr,
+                            float *yi, long ldy, long lines, float scale, hipStream_t s) const
+{
+  if (lines <= 0) return OVX_OK;
+  if (kind != HIER) return fail(OVX_ERR_UNSUPPORTED, "split-storage fastconv runs natively on the hierarchical path");
+  if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
+  HierArgs a = hier_args(*this);
+  a.xr = xr;
+  a.xi = xi;
+  a.yr = yr;
+  a.yi = yi;
+  return hier_fastconv(a, HIER_FASTCONV_SPLIT, ldx, w, ldw, ldy, lines, scale, s);
+}
+
 int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, float2 *y, long ldy, long lines,
                       float scale, hipStream_t s) const
 {
   if (lines <= 0) return OVX_OK;
   if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
   if (kind == HIER) {
-    if (!offsets_fit(hier_rows_per_block(n), std::max(std::max(ldx, ldy), ldw)))
-      return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
     HierArgs a = hier_args(*this);
     a.x = x;
-    a.ldx = ldx;
     a.y = y;
-    a.ldy = ldy;
-    a.rows = lines;
-    if (ldw == 0) {
-      // one spectrum: permute it once into the kernel's [k1][k2] order with
-      // the scale folded in
-      int st = wp.alloc((size_t)n * 8);
-      if (st) return st;
-      hipError_t e = launch_hier_permute(w, 0, (float2 *)wp.p, n, 1, scale, s);
-      if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
-      a.w = (const float2 *)wp.p;
-      a.ldw = 0;
-      a.natural = 0;
-      a.scale = 1.0f;
-    } else {
-      a.w = w;
-      a.ldw = ldw;
-      a.natural = 1;
-      a.scale = scale;
-    }
-    hipError_t e = launch_hier(HIER_FASTCONV, n, a, s);
-    if (e != hipSuccess) return hip_fail(e, "fastconv launch");
-    return OVX_OK;
+    return hier_fastconv(a, HIER_FASTCONV, ldx, w, ldw, ldy, lines, scale, s);
   }
   if (kind == LDS) {
     long ld = std::max(std::max(ldx, ldy), ldw);
@@ -780,10 +800,12 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
 // Split-complex storage (OpenVSIP's split_complex format: real and imaginary
 // parts in two float arrays with the same strides), the fftm_backend
 // overloads on std::pair<T*, T*> (backend.hpp:741-783; the FFTW backend's
-// split DFTs, fftw.cpp:945-1001).  The lines are interleaved into the plan's
-// workspace, the interleaved path runs there in place, and the result is
-// split into the output arrays: two extra passes over the data, and the same
-// floats as the interleaved call on the same values.
+// split DFTs, fftw.cpp:945-1001).  The hierarchical fastconv kernel reads
+// and writes split arrays itself (two 32-bit accesses per value instead of
+// one 64-bit one).  Everything else goes through the plan's workspace: the
+// lines are interleaved, the interleaved path runs there in place, and the
+// result is split into the output arrays -- two extra passes over the data.
+// Either way the floats equal the interleaved call's on the same values.
 namespace {
 
 struct SplitIn {
@@ -888,6 +910,16 @@ extern "C" int ovx_fastconv_split_exec(ovx_plan p, const void *x_re, const void 
   hipStream_t s = (hipStream_t)stream;
   const long wlines = ldw ? rows : 1;
   auto run = [&](SplitIn x, SplitIn w, SplitOut y) -> int {
+    if (p->line.native_split()) {
+      // the kernel reads and writes the split arrays itself; only the
+      // spectrum is interleaved first
+      int r = p->split_ws.alloc((size_t)wlines * n * 8);
+      if (r) return r;
+      float2 *ww = (float2 *)p->split_ws.p;
+      hipError_t e = launch_interleave(w.re, w.im, ldw, ww, n, wlines, n, s);
+      if (e != hipSuccess) return hip_fail(e, "interleave");
+      return p->line.fastconv_split(x.re, x.im, ldx, ww, ldw ? n : 0, y.re, y.im, ldy, rows, p->scale, s);
+    }
     int r = p->split_ws.alloc((size_t)(rows + wlines) * n * 8);
     if (r) return r;
     float2 *wx = (float2 *)p->split_ws.p, *ww = wx + (size_t)rows * n;

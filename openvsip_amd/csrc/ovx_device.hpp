@@ -195,5 +195,37 @@ __device__ __forceinline__ void bstore_nt(rsrc_t r, float2 v, uint32_t voff, uin
 {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 2);
 }
+__device__ __forceinline__ float bload1_nt(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+}
+__device__ __forceinline__ void bstore1_nt(rsrc_t r, float v, uint32_t voff, uint32_t soff)
+{
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 2);
+}
+
+// Streaming access to one line of complex values, element e at a per-lane
+// base plus a constant offset.  IoC: interleaved complex<float> (one 64-bit
+// access per value).  IoS: split storage, real and imaginary parts in two
+// float arrays with the same layout (two 32-bit accesses, each coalesced).
+struct IoC {
+  rsrc_t r;
+  uint32_t v; // lane base, bytes
+  __device__ __forceinline__ float2 ld(uint32_t e) const { return bload_nt(r, v, e * 8u); }
+  __device__ __forceinline__ void st(float2 x, uint32_t e) const { bstore_nt(r, x, v, e * 8u); }
+};
+struct IoS {
+  rsrc_t re, im;
+  uint32_t v; // lane base, bytes
+  __device__ __forceinline__ float2 ld(uint32_t e) const
+  {
+    return make_float2(bload1_nt(re, v, e * 4u), bload1_nt(im, v, e * 4u));
+  }
+  __device__ __forceinline__ void st(float2 x, uint32_t e) const
+  {
+    bstore1_nt(re, x.x, v, e * 4u);
+    bstore1_nt(im, x.y, v, e * 4u);
+  }
+};
 
 } // namespace ovx

@@ -21,7 +21,7 @@
 namespace ovx {
 
 // ---------------------------------------------------------------- kernels
-template <int N>
+template <int N, bool SPLIT>
 __global__ __launch_bounds__(Hier<N>::NT) void k_hier_fastconv(HierArgs a)
 {
   using H = Hier<N>;
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(Hier<N>::NT) void k_hier_fastconv(HierArgs a)
   const int lrow = (H::B == 1) ? 0 : (int)threadIdx.x / H::TR;
   const int t = (H::B == 1) ? (int)threadIdx.x : (int)threadIdx.x % H::TR;
   // one row per block and a grid of exactly `rows` blocks: every row exists
-  hier_fastconv_body<N, H::B == 1>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
+  hier_fastconv_body<N, H::B == 1, SPLIT>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
 }
 
 template <int N>
@@ -82,7 +82,8 @@ static hipError_t launch_kind(int kind, const HierArgs &a, hipStream_t s)
 {
   using H = Hier<N>;
   dim3 grid((unsigned)((a.rows + H::B - 1) / H::B)), block(H::NT);
-  if (kind == HIER_FASTCONV) hipLaunchKernelGGL(k_hier_fastconv<N>, grid, block, 0, s, a);
+  if (kind == HIER_FASTCONV) hipLaunchKernelGGL((k_hier_fastconv<N, false>), grid, block, 0, s, a);
+  else if (kind == HIER_FASTCONV_SPLIT) hipLaunchKernelGGL((k_hier_fastconv<N, true>), grid, block, 0, s, a);
   else if (kind == HIER_FWD) hipLaunchKernelGGL(k_hier_fft_fwd<N>, grid, block, 0, s, a);
   else hipLaunchKernelGGL(k_hier_fft_inv<N>, grid, block, 0, s, a);
   return hipGetLastError();

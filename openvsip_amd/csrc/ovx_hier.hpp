@@ -5,7 +5,7 @@
 
 namespace ovx {
 
-enum { HIER_FASTCONV = 0, HIER_FWD = 1, HIER_INV = 2 };
+enum { HIER_FASTCONV = 0, HIER_FWD = 1, HIER_INV = 2, HIER_FASTCONV_SPLIT = 3 };
 
 struct HierArgs {
   const float2 *x;
@@ -19,6 +19,10 @@ struct HierArgs {
   const float2 *tw_wave;  // 1024-point engine table
   float scale;            // applied unless 1
   int natural;            // spectrum in natural order (gathered)
+  // HIER_FASTCONV_SPLIT: X and Y as split real / imaginary arrays (line
+  // strides ldx, ldy in floats); x and y are unused
+  const float *xr = nullptr, *xi = nullptr;
+  float *yr = nullptr, *yi = nullptr;
 };
 
 bool hier_supported(long n);

@@ -23,9 +23,8 @@ struct Hier {
 };
 
 // Outer forward stage: load, DFT_R0, twiddle, write slices.
-template <int N, typename LP>
-__device__ __forceinline__ void outer_forward(const rsrc_t &rx, uint32_t vx, bool valid, LP lds, const rsrc_t &rto,
-                                              int t)
+template <int N, typename IO, typename LP>
+__device__ __forceinline__ void outer_forward(const IO &ix, bool valid, LP lds, const rsrc_t &rto, int t)
 {
   using H = Hier<N>;
   constexpr int R0 = H::R0, M0 = H::M0, NW = H::NW;
@@ -35,7 +34,7 @@ __device__ __forceinline__ void outer_forward(const rsrc_t &rx, uint32_t vx, boo
     float2 a[R0];
 #pragma unroll
     for (int r = 0; r < R0; ++r)
-      a[r] = valid ? bload_nt(rx, vx, (uint32_t)(H::TR * m + NW * r) * 8u) : make_float2(0.f, 0.f);
+      a[r] = valid ? ix.ld((uint32_t)(H::TR * m + NW * r)) : make_float2(0.f, 0.f);
     if constexpr (R0 > 1) {
       Dft<R0, -1>::run(a);
 #pragma unroll
@@ -47,9 +46,8 @@ __device__ __forceinline__ void outer_forward(const rsrc_t &rx, uint32_t vx, boo
 }
 
 // Outer inverse stage: read slices, DFT_R0 (inverse), scale, store.
-template <int N, typename LP>
-__device__ __forceinline__ void outer_inverse_store(const rsrc_t &ry, uint32_t vy, bool valid, LP lds,
-                                                    int t, float scale)
+template <int N, typename IO, typename LP>
+__device__ __forceinline__ void outer_inverse_store(const IO &iy, bool valid, LP lds, int t, float scale)
 {
   using H = Hier<N>;
   constexpr int R0 = H::R0, M0 = H::M0, NW = H::NW;
@@ -66,7 +64,7 @@ __device__ __forceinline__ void outer_inverse_store(const rsrc_t &ry, uint32_t v
     }
     if (valid) {
 #pragma unroll
-      for (int r = 0; r < R0; ++r) bstore_nt(ry, a[r], vy, (uint32_t)(H::TR * m + NW * r) * 8u);
+      for (int r = 0; r < R0; ++r) iy.st(a[r], (uint32_t)(H::TR * m + NW * r));
     }
   }
 }
@@ -81,23 +79,35 @@ __device__ __forceinline__ void outer_inverse_store(const rsrc_t &ry, uint32_t v
 // barriers); rows at or beyond a.rows compute on zeros and store nothing.
 // ONE_ROW: the caller's workgroup holds a single row that exists (one row
 // per block, one block per row), so no row needs the check.
-template <int N, bool ONE_ROW = false, typename LP>
+// SPLIT: X and Y in split storage (a.xr/a.xi, a.yr/a.yi); the spectrum is
+// interleaved either way.
+template <int N, bool ONE_ROW = false, bool SPLIT = false, typename LP>
 __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0, int lrow, int t, LP lds)
 {
   using H = Hier<N>;
   constexpr int R0 = H::R0, NW = H::NW;
   const long row = row0 + lrow;
   const bool valid = ONE_ROW || row < a.rows;
-  const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
-  const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
+  const auto ix = [&] {
+    if constexpr (SPLIT)
+      return IoS{make_rsrc(a.xr + row0 * a.ldx, 0xffffffffu), make_rsrc(a.xi + row0 * a.ldx, 0xffffffffu),
+                 (uint32_t)(lrow * a.ldx + t) * 4u};
+    else
+      return IoC{make_rsrc(a.x + row0 * a.ldx, 0xffffffffu), (uint32_t)(lrow * a.ldx + t) * 8u};
+  }();
+  const auto iy = [&] {
+    if constexpr (SPLIT)
+      return IoS{make_rsrc(a.yr + row0 * a.ldy, 0xffffffffu), make_rsrc(a.yi + row0 * a.ldy, 0xffffffffu),
+                 (uint32_t)(lrow * a.ldy + t) * 4u};
+    else
+      return IoC{make_rsrc(a.y + row0 * a.ldy, 0xffffffffu), (uint32_t)(lrow * a.ldy + t) * 8u};
+  }();
   const rsrc_t rw = make_rsrc(a.w + row0 * a.ldw, 0xffffffffu);
   const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
   const rsrc_t rto = make_rsrc(a.tw_outer, (uint32_t)((R0 - 1) * H::NW) * 8u); // outer twiddles
-  const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
-  const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
 
   if constexpr (R0 > 1) {
-    outer_forward<N, LP>(rx, vx, valid, lds, rto, t);
+    outer_forward<N>(ix, valid, lds, rto, t);
     __syncthreads();
   }
   // ---- wave stage: wave k1 owns slice k1
@@ -112,8 +122,7 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
     for (int s = 0; s < 16; ++s) u[s] = lds_ld(sl, ls + s * H::WE::Layout::step(64));
   } else {
 #pragma unroll
-    for (int s = 0; s < 16; ++s)
-      u[s] = valid ? bload_nt(rx, vx, (uint32_t)(64 * s) * 8u) : make_float2(0.f, 0.f);
+    for (int s = 0; s < 16; ++s) u[s] = valid ? ix.ld((uint32_t)(64 * s)) : make_float2(0.f, 0.f);
   }
   H::WE::template fft<-1>(u, sl, l, rtw);
   if (a.natural) {
@@ -145,11 +154,11 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
 #pragma unroll
     for (int s = 0; s < 16; ++s) lds_st(sl, ls + s * H::WE::Layout::step(64), u[s]);
     __syncthreads();
-    outer_inverse_store<N, LP>(ry, vy, valid, lds, t, 1.0f);
+    outer_inverse_store<N>(iy, valid, lds, t, 1.0f);
   } else {
     if (valid) {
 #pragma unroll
-      for (int s = 0; s < 16; ++s) bstore_nt(ry, u[s], vy, (uint32_t)(64 * s) * 8u);
+      for (int s = 0; s < 16; ++s) iy.st(u[s], (uint32_t)(64 * s));
     }
   }
 }
@@ -172,7 +181,7 @@ __device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int 
   const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
 
   if constexpr (R0 > 1) {
-    outer_forward<N, LP>(rx, vx, valid, lds, rto, t);
+    outer_forward<N>(IoC{rx, vx}, valid, lds, rto, t);
     __syncthreads();
   }
   const int k1 = t >> 6;
@@ -255,7 +264,7 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
 #pragma unroll
     for (int s = 0; s < 16; ++s) lds_st(sl, ls + s * H::WE::Layout::step(64), u[s]);
     __syncthreads();
-    outer_inverse_store<N, LP>(ry, vy, valid, lds, t, a.scale);
+    outer_inverse_store<N>(IoC{ry, vy}, valid, lds, t, a.scale);
   } else {
     if (a.scale != 1.0f) {
 #pragma unroll

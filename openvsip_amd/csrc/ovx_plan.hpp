@@ -5,6 +5,7 @@
 #include <stddef.h>
 
 #include "../../include/ovxhip.h"
+#include "ovx_hier.hpp"
 #include "ovx_internal.hpp"
 
 namespace ovx {
@@ -54,6 +55,14 @@ struct LineFFT {
                    float2 *ws, hipStream_t s) const;
   int fastconv(const float2 *x, long ldx, const float2 *w, long ldw, float2 *y, long ldy, long lines, float scale,
                hipStream_t s) const;
+  // fastconv with X and Y in split storage (line strides in floats) and an
+  // interleaved spectrum, read and written by the kernel itself; only the
+  // hierarchical path (native_split())
+  bool native_split() const { return kind == HIER; }
+  int fastconv_split(const float *xr, const float *xi, long ldx, const float2 *w, long ldw, float *yr, float *yi,
+                     long ldy, long lines, float scale, hipStream_t s) const;
+  int hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *w, long ldw, long ldy, long lines,
+                    float scale, hipStream_t s) const;
   // Spectrum layout the fastconv kernels of this path read: wp = scale * w
   // permuted for the path (n values).  fastconv_prepared() takes such a
   // spectrum (one vector for all lines) and applies no further scale.
