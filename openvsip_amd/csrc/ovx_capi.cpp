@@ -337,21 +337,35 @@ int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, floa
     if (st) return st;
     hipError_t e = launch_fs_permute(w, ldw, (float2 *)wp.p, n1, n2, wrows, scale, s);
     if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
-    FsArgs a = fs_args(*this);
-    a.wp = (const float2 *)wp.p;
-    a.ldwp = ldw ? n : 0;
-    a.in = x;
-    a.ld_in = ldx;
-    a.out = y;
-    a.ld_out = ldy;
-    if ((e = launch_fs_col(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
-    a.in = y;
-    a.ld_in = ldy;
-    if ((e = launch_fs_row(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step row pass");
-    if ((e = launch_fs_col(1, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
-    return OVX_OK;
+    return fs_fastconv(x, ldx, (const float2 *)wp.p, ldw ? n : 0, y, ldy, lines, s);
   }
   return fail(OVX_ERR_UNSUPPORTED, "no fastconv path");
+}
+
+// Four-step fastconv, three passes over all rows: column FFT + twiddle
+// (X -> Y), row FFT x spectrum, inverse row FFT, inverse twiddle (Y in
+// place), inverse column FFT (Y in place).  Running the passes chunk by chunk
+// of rows, to keep a chunk's Y in the Infinity Cache between passes, measured
+// slower at every chunk size from 16 to 128 MiB of X (the passes lose
+// parallelism faster than they gain cache hits), so one launch per pass
+// covers every row.
+int LineFFT::fs_fastconv(const float2 *x, long ldx, const float2 *wprep, long ldwp, float2 *y, long ldy, long lines,
+                         hipStream_t s) const
+{
+  FsArgs a = fs_args(*this);
+  a.wp = wprep;
+  a.ldwp = ldwp;
+  a.in = x;
+  a.ld_in = ldx;
+  a.out = y;
+  a.ld_out = ldy;
+  hipError_t e;
+  if ((e = launch_fs_col(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
+  a.in = y;
+  a.ld_in = ldy;
+  if ((e = launch_fs_row(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step row pass");
+  if ((e = launch_fs_col(1, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
+  return OVX_OK;
 }
 
 int LineFFT::prepare_spectrum(const float2 *w, float2 *wp_out, float scale, hipStream_t s) const
@@ -388,20 +402,7 @@ int LineFFT::fastconv_prepared(const float2 *x, long ldx, const float2 *wprep, f
   }
   if (kind == FOURSTEP) {
     if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step fastconv");
-    FsArgs a = fs_args(*this);
-    a.wp = wprep;
-    a.ldwp = 0;
-    a.in = x;
-    a.ld_in = ldx;
-    a.out = y;
-    a.ld_out = ldy;
-    hipError_t e;
-    if ((e = launch_fs_col(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
-    a.in = y;
-    a.ld_in = ldy;
-    if ((e = launch_fs_row(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step row pass");
-    if ((e = launch_fs_col(1, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
-    return OVX_OK;
+    return fs_fastconv(x, ldx, wprep, 0, y, ldy, lines, s);
   }
   return fastconv(x, ldx, wprep, 0, y, ldy, lines, 1.0f, s);
 }

@@ -63,6 +63,8 @@ struct LineFFT {
                      long ldy, long lines, float scale, hipStream_t s) const;
   int hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *w, long ldw, long ldy, long lines,
                     float scale, hipStream_t s) const;
+  int fs_fastconv(const float2 *x, long ldx, const float2 *wprep, long ldwp, float2 *y, long ldy, long lines,
+                  hipStream_t s) const;
   // Spectrum layout the fastconv kernels of this path read: wp = scale * w
   // permuted for the path (n values).  fastconv_prepared() takes such a
   // spectrum (one vector for all lines) and applies no further scale.
