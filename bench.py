@@ -378,6 +378,9 @@ def serve_section(args):
             y = torch.empty_like(x)
             rep = reps[n][int(rng.integers(len(reps[n])))] if kind == "fc" else None
             pool[kind].append((x, y, rep))
+    # the server's streams do not wait for torch's: the inputs are written
+    # before any request goes in
+    torch.cuda.synchronize()
 
     def make_trace(count, rate):
         t = np.cumsum(rng.exponential(1.0 / rate, size=count)) if rate else np.zeros(count)

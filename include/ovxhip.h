@@ -304,7 +304,10 @@ int ovx_batch_destroy(ovx_batch batch);
    arrive meanwhile go into the next free slot, so batches overlap on the
    device.  Data pointers are device pointers owned by the caller and must
    stay valid until the request completes, and the replicas and correlation
-   references a request names until it completes.  plain != 0 runs every
+   references a request names until it completes.  The server's streams do
+   not wait for work the caller queued on other streams: x must be written,
+   and nothing else may still be writing y, when the request is submitted.
+   plain != 0 runs every
    request as its own standalone launch instead (the per-request path).
    ovx_server_destroy waits for the batches already launched; requests still
    queued are dropped. */

@@ -448,7 +448,9 @@ def corr_request(x, reference, y, bias="biased", t_arrival=0.0):
 
 class Server:
     """Continuous-batching server (ovx_server_*); plain=True runs every request
-    as its own standalone launch."""
+    as its own standalone launch.  The server runs on its own streams, which
+    do not wait for torch's: synchronize the stream that wrote a request's
+    input (or is still writing its output) before submitting it."""
 
     def __init__(self, slots=3, max_jobs=512, max_tiles=65536, hbm_budget_mb=64 * 1024, workspace_mb=1024,
                  plain=False):

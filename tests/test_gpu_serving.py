@@ -51,7 +51,7 @@ def build_mix(ovx):
 def run(ovx, items, plain):
     import torch
     srv = ovx.Server(slots=3, max_jobs=8, hbm_budget_mb=4096, workspace_mb=256, plain=plain)
-    outs, tickets = [], []
+    outs, reqs = [], []
     for it in items:
         if it["kind"] == "corr":
             y = torch.full((it["cref"].output_size,), float("nan"), dtype=torch.complex64, device="cuda")
@@ -60,8 +60,12 @@ def run(ovx, items, plain):
             y = torch.full_like(it["xd"], float("nan"))
             req = ovx.fastconv_request(it["xd"], it["rep"][0], y) if it["kind"] == "fc" else \
                 ovx.fftm_request(it["xd"], y)
-        tickets.append(srv.submit(req))
+        reqs.append(req)
         outs.append(y)
+    # the server's streams do not wait for torch's: the inputs and the NaN
+    # fills of the outputs must have landed before the requests go in
+    torch.cuda.synchronize()
+    tickets = [srv.submit(req) for req in reqs]
     lats = [srv.wait(t) for t in tickets]
     res = [host(y) for y in outs]
     srv.close()
@@ -112,6 +116,7 @@ def test_server_close_with_requests_in_flight(ovx):
     rep = ovx.Replica(16384, w, is_spectrum=True)
     xs = [torch.randn((64, 16384), dtype=torch.complex64, device="cuda") for _ in range(4)]
     ys = [torch.empty_like(x) for x in xs]
+    torch.cuda.synchronize()  # the inputs are written before the server reads them
     for i in range(40):
         srv.submit(ovx.fastconv_request(xs[i % 4], rep, ys[i % 4]))
     t0 = time.time()
@@ -146,8 +151,7 @@ def test_server_large_four_step_batch(ovx):
         # next free slot takes them as one batch
         busy_x = torch.randn((256, 65536), dtype=torch.complex64, device="cuda")
         busy_y = [torch.empty_like(busy_x) for _ in range(3)]
-        busy = [srv.submit(ovx.fastconv_request(busy_x, rep, y)) for y in busy_y]
-        outs, tickets = [], []
+        outs, reqs = [], []
         for kind, x in xs:
             y = torch.full_like(x, float("nan"))
             if kind == "fc":
@@ -156,8 +160,13 @@ def test_server_large_four_step_batch(ovx):
                 req = ovx.fftm_request(x, y)
             else:
                 req = ovx.fastconv_request(x, rep2, y)
-            tickets.append(srv.submit(req))
+            reqs.append(req)
             outs.append(y)
+        # the server's streams do not wait for torch's: every input and NaN
+        # fill must have landed before the requests go in
+        torch.cuda.synchronize()
+        busy = [srv.submit(ovx.fastconv_request(busy_x, rep, y)) for y in busy_y]
+        tickets = [srv.submit(req) for req in reqs]
         for t in busy + tickets:
             srv.wait(t)
         res = [host(y) for y in outs]
