@@ -297,6 +297,27 @@ int LineFFT::hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *
   return e == hipSuccess ? OVX_OK : hip_fail(e, "fastconv launch");
 }
 
+int LineFFT::transform_split(int dir, const float *in_re, const float *in_im, long ld_in, float *out_re,
+                             float *out_im, long ld_out, long lines, float scale, hipStream_t s) const
+{
+  if (lines <= 0) return OVX_OK;
+  if (kind != HIER) return fail(OVX_ERR_UNSUPPORTED, "split-storage transforms run natively on the hierarchical path");
+  if (!tables_ready()) return fail(OVX_ERR_STATE, "transform plan has no twiddle tables");
+  if (!offsets_fit(hier_rows_per_block(n), std::max(ld_in, ld_out)))
+    return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
+  HierArgs a = hier_args(*this);
+  a.xr = in_re;
+  a.xi = in_im;
+  a.ldx = ld_in;
+  a.yr = out_re;
+  a.yi = out_im;
+  a.ldy = ld_out;
+  a.rows = lines;
+  a.scale = scale;
+  hipError_t e = launch_hier(dir < 0 ? HIER_FWD_SPLIT : HIER_INV_SPLIT, n, a, s);
+  return e == hipSuccess ? OVX_OK : hip_fail(e, "row fft launch");
+}
+
 int LineFFT::fastconv_split(const float *xr, const float *xi, long ldx, const float2 *w, long ldw, float *yr,
                             float *yi, long ldy, long lines, float scale, hipStream_t s) const
 {
@@ -801,9 +822,10 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
 // Split-complex storage (OpenVSIP's split_complex format: real and imaginary
 // parts in two float arrays with the same strides), the fftm_backend
 // overloads on std::pair<T*, T*> (backend.hpp:741-783; the FFTW backend's
-// split DFTs, fftw.cpp:945-1001).  The hierarchical fastconv kernel reads
-// and writes split arrays itself (two 32-bit accesses per value instead of
-// one 64-bit one).  Everything else goes through the plan's workspace: the
+// split DFTs, fftw.cpp:945-1001).  The hierarchical kernels (fastconv and
+// transforms) read and write split arrays themselves (two 32-bit accesses
+// per value instead of one 64-bit one).  Everything else goes through the
+// plan's workspace: the
 // lines are interleaved, the interleaved path runs there in place, and the
 // result is split into the output arrays -- two extra passes over the data.
 // Either way the floats equal the interleaved call's on the same values.
@@ -842,6 +864,8 @@ static int fftm_split_exec(ovx_plan p, const void *in_re, const void *in_im, lon
   hipStream_t s = (hipStream_t)stream;
   const int dir = sign_of(p->dir);
   auto run = [&](SplitIn a, SplitOut b) -> int {
+    if (p->line.native_split())
+      return p->line.transform_split(dir, a.re, a.im, ldi, b.re, b.im, ldo, lines, p->scale, s);
     int r = p->split_ws.alloc((size_t)lines * n * 8);
     if (r) return r;
     float2 *ws = (float2 *)p->split_ws.p;

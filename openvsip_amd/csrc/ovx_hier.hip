@@ -32,24 +32,24 @@ __global__ __launch_bounds__(Hier<N>::NT) void k_hier_fastconv(HierArgs a)
   hier_fastconv_body<N, H::B == 1, SPLIT>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
 }
 
-template <int N>
+template <int N, bool SPLIT>
 __global__ __launch_bounds__(Hier<N>::NT) void k_hier_fft_fwd(HierArgs a)
 {
   using H = Hier<N>;
   __shared__ float2 lds_all[H::B * H::LROW];
   const int lrow = (H::B == 1) ? 0 : (int)threadIdx.x / H::TR;
   const int t = (H::B == 1) ? (int)threadIdx.x : (int)threadIdx.x % H::TR;
-  hier_fwd_body<N>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
+  hier_fwd_body<N, SPLIT>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
 }
 
-template <int N>
+template <int N, bool SPLIT>
 __global__ __launch_bounds__(Hier<N>::NT) void k_hier_fft_inv(HierArgs a)
 {
   using H = Hier<N>;
   __shared__ float2 lds_all[H::B * H::LROW];
   const int lrow = (H::B == 1) ? 0 : (int)threadIdx.x / H::TR;
   const int t = (H::B == 1) ? (int)threadIdx.x : (int)threadIdx.x % H::TR;
-  hier_inv_body<N>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
+  hier_inv_body<N, SPLIT>(a, (long)blockIdx.x * H::B, lrow, t, lds_all + lrow * H::LROW);
 }
 
 // Spectrum permutation to [k1][k2] order with scale folded in:
@@ -84,8 +84,11 @@ static hipError_t launch_kind(int kind, const HierArgs &a, hipStream_t s)
   dim3 grid((unsigned)((a.rows + H::B - 1) / H::B)), block(H::NT);
   if (kind == HIER_FASTCONV) hipLaunchKernelGGL((k_hier_fastconv<N, false>), grid, block, 0, s, a);
   else if (kind == HIER_FASTCONV_SPLIT) hipLaunchKernelGGL((k_hier_fastconv<N, true>), grid, block, 0, s, a);
-  else if (kind == HIER_FWD) hipLaunchKernelGGL(k_hier_fft_fwd<N>, grid, block, 0, s, a);
-  else hipLaunchKernelGGL(k_hier_fft_inv<N>, grid, block, 0, s, a);
+  else if (kind == HIER_FWD) hipLaunchKernelGGL((k_hier_fft_fwd<N, false>), grid, block, 0, s, a);
+  else if (kind == HIER_INV) hipLaunchKernelGGL((k_hier_fft_inv<N, false>), grid, block, 0, s, a);
+  else if (kind == HIER_FWD_SPLIT) hipLaunchKernelGGL((k_hier_fft_fwd<N, true>), grid, block, 0, s, a);
+  else if (kind == HIER_INV_SPLIT) hipLaunchKernelGGL((k_hier_fft_inv<N, true>), grid, block, 0, s, a);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

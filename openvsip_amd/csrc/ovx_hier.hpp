@@ -5,7 +5,7 @@
 
 namespace ovx {
 
-enum { HIER_FASTCONV = 0, HIER_FWD = 1, HIER_INV = 2, HIER_FASTCONV_SPLIT = 3 };
+enum { HIER_FASTCONV = 0, HIER_FWD = 1, HIER_INV = 2, HIER_FASTCONV_SPLIT = 3, HIER_FWD_SPLIT = 4, HIER_INV_SPLIT = 5 };
 
 struct HierArgs {
   const float2 *x;

@@ -22,6 +22,27 @@ struct Hier {
   static_assert(R0 * NW == N && R0 >= 1 && R0 <= 16, "N must be 1024 * {1,2,4,8,16}");
 };
 
+// Row accessors of the input and output lines of rows row0 + lrow, thread t:
+// interleaved (a.x, a.y) or split storage (a.xr/a.xi, a.yr/a.yi).
+template <bool SPLIT>
+__device__ __forceinline__ auto hier_in(const HierArgs &a, long row0, int lrow, int t)
+{
+  if constexpr (SPLIT)
+    return IoS{make_rsrc(a.xr + row0 * a.ldx, 0xffffffffu), make_rsrc(a.xi + row0 * a.ldx, 0xffffffffu),
+               (uint32_t)(lrow * a.ldx + t) * 4u};
+  else
+    return IoC{make_rsrc(a.x + row0 * a.ldx, 0xffffffffu), (uint32_t)(lrow * a.ldx + t) * 8u};
+}
+template <bool SPLIT>
+__device__ __forceinline__ auto hier_out(const HierArgs &a, long row0, int lrow, int t)
+{
+  if constexpr (SPLIT)
+    return IoS{make_rsrc(a.yr + row0 * a.ldy, 0xffffffffu), make_rsrc(a.yi + row0 * a.ldy, 0xffffffffu),
+               (uint32_t)(lrow * a.ldy + t) * 4u};
+  else
+    return IoC{make_rsrc(a.y + row0 * a.ldy, 0xffffffffu), (uint32_t)(lrow * a.ldy + t) * 8u};
+}
+
 // Outer forward stage: load, DFT_R0, twiddle, write slices.
 template <int N, typename IO, typename LP>
 __device__ __forceinline__ void outer_forward(const IO &ix, bool valid, LP lds, const rsrc_t &rto, int t)
@@ -88,20 +109,8 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
   constexpr int R0 = H::R0, NW = H::NW;
   const long row = row0 + lrow;
   const bool valid = ONE_ROW || row < a.rows;
-  const auto ix = [&] {
-    if constexpr (SPLIT)
-      return IoS{make_rsrc(a.xr + row0 * a.ldx, 0xffffffffu), make_rsrc(a.xi + row0 * a.ldx, 0xffffffffu),
-                 (uint32_t)(lrow * a.ldx + t) * 4u};
-    else
-      return IoC{make_rsrc(a.x + row0 * a.ldx, 0xffffffffu), (uint32_t)(lrow * a.ldx + t) * 8u};
-  }();
-  const auto iy = [&] {
-    if constexpr (SPLIT)
-      return IoS{make_rsrc(a.yr + row0 * a.ldy, 0xffffffffu), make_rsrc(a.yi + row0 * a.ldy, 0xffffffffu),
-                 (uint32_t)(lrow * a.ldy + t) * 4u};
-    else
-      return IoC{make_rsrc(a.y + row0 * a.ldy, 0xffffffffu), (uint32_t)(lrow * a.ldy + t) * 8u};
-  }();
+  const auto ix = hier_in<SPLIT>(a, row0, lrow, t);
+  const auto iy = hier_out<SPLIT>(a, row0, lrow, t);
   const rsrc_t rw = make_rsrc(a.w + row0 * a.ldw, 0xffffffffu);
   const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
   const rsrc_t rto = make_rsrc(a.tw_outer, (uint32_t)((R0 - 1) * H::NW) * 8u); // outer twiddles
@@ -166,22 +175,20 @@ __device__ __forceinline__ void hier_fastconv_body(const HierArgs &a, long row0,
 // ---------------------------------------------------------------- transforms
 // Forward: outer stage, wave FFT, then the natural-order result
 // X[k1 + R0 k2] is staged through LDS so that the store is coalesced.
-template <int N, typename LP>
+template <int N, bool SPLIT = false, typename LP>
 __device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int lrow, int t, LP lds)
 {
   using H = Hier<N>;
   constexpr int R0 = H::R0;
   const long row = row0 + lrow;
   const bool valid = row < a.rows;
-  const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
-  const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
+  const auto ix = hier_in<SPLIT>(a, row0, lrow, t);
+  const auto iy = hier_out<SPLIT>(a, row0, lrow, t);
   const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
   const rsrc_t rto = make_rsrc(a.tw_outer, (uint32_t)((R0 - 1) * H::NW) * 8u); // outer twiddles
-  const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
-  const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
 
   if constexpr (R0 > 1) {
-    outer_forward<N>(IoC{rx, vx}, valid, lds, rto, t);
+    outer_forward<N>(ix, valid, lds, rto, t);
     __syncthreads();
   }
   const int k1 = t >> 6;
@@ -196,7 +203,7 @@ __device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int 
   } else {
 #pragma unroll
     for (int s = 0; s < 16; ++s)
-      u[s] = valid ? bload_nt(rx, vx, (uint32_t)(64 * s) * 8u) : make_float2(0.f, 0.f);
+      u[s] = valid ? ix.ld((uint32_t)(64 * s)) : make_float2(0.f, 0.f);
   }
   H::WE::template fft<-1>(u, sl, l, rtw);
   if (a.scale != 1.0f) {
@@ -210,31 +217,29 @@ __device__ __forceinline__ void hier_fwd_body(const HierArgs &a, long row0, int 
     __syncthreads();
     if (valid) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) bstore_nt(ry, lds_ld(lds, H::lidx(t + H::TR * q)), vy, (uint32_t)(H::TR * q) * 8u);
+      for (int q = 0; q < 16; ++q) iy.st(lds_ld(lds, H::lidx(t + H::TR * q)), (uint32_t)(H::TR * q));
     }
   } else {
     if (valid) {
 #pragma unroll
-      for (int s = 0; s < 16; ++s) bstore_nt(ry, u[s], vy, (uint32_t)(64 * s) * 8u);
+      for (int s = 0; s < 16; ++s) iy.st(u[s], (uint32_t)(64 * s));
     }
   }
 }
 
 // Inverse: natural-order input staged through LDS into [k1][k2] slices, wave
 // inverse FFT, inverse twiddle, outer inverse stage.
-template <int N, typename LP>
+template <int N, bool SPLIT = false, typename LP>
 __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int lrow, int t, LP lds)
 {
   using H = Hier<N>;
   constexpr int R0 = H::R0, NW = H::NW;
   const long row = row0 + lrow;
   const bool valid = row < a.rows;
-  const rsrc_t rx = make_rsrc(a.x + row0 * a.ldx, 0xffffffffu);
-  const rsrc_t ry = make_rsrc(a.y + row0 * a.ldy, 0xffffffffu);
+  const auto ix = hier_in<SPLIT>(a, row0, lrow, t);
+  const auto iy = hier_out<SPLIT>(a, row0, lrow, t);
   const rsrc_t rtw = make_rsrc(a.tw_wave, (uint32_t)(H::WE::plan.twtotal + 1) * 8u);
   const rsrc_t rto = make_rsrc(a.tw_outer, (uint32_t)((R0 - 1) * H::NW) * 8u); // outer twiddles
-  const uint32_t vx = (uint32_t)(lrow * a.ldx + t) * 8u;
-  const uint32_t vy = (uint32_t)(lrow * a.ldy + t) * 8u;
   const int k1 = t >> 6;
   const int l = t & 63;
   const int ls = H::WE::Layout::idx(l); // slot of position l; l + 64 s is at ls + s * step(64)
@@ -244,7 +249,7 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
   if constexpr (R0 > 1) {
 #pragma unroll
     for (int q = 0; q < 16; ++q)
-      lds_st(lds, H::lidx(t + H::TR * q), valid ? bload_nt(rx, vx, (uint32_t)(H::TR * q) * 8u) : make_float2(0.f, 0.f));
+      lds_st(lds, H::lidx(t + H::TR * q), valid ? ix.ld((uint32_t)(H::TR * q)) : make_float2(0.f, 0.f));
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 16; ++s) u[s] = lds_ld(lds, H::lidx(k1 + R0 * (l + 64 * s)));
@@ -252,7 +257,7 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
   } else {
 #pragma unroll
     for (int s = 0; s < 16; ++s)
-      u[s] = valid ? bload_nt(rx, vx, (uint32_t)(64 * s) * 8u) : make_float2(0.f, 0.f);
+      u[s] = valid ? ix.ld((uint32_t)(64 * s)) : make_float2(0.f, 0.f);
   }
   H::WE::template fft<+1>(u, sl, l, rtw);
   if constexpr (R0 > 1) {
@@ -264,7 +269,7 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
 #pragma unroll
     for (int s = 0; s < 16; ++s) lds_st(sl, ls + s * H::WE::Layout::step(64), u[s]);
     __syncthreads();
-    outer_inverse_store<N>(IoC{ry, vy}, valid, lds, t, a.scale);
+    outer_inverse_store<N>(iy, valid, lds, t, a.scale);
   } else {
     if (a.scale != 1.0f) {
 #pragma unroll
@@ -272,7 +277,7 @@ __device__ __forceinline__ void hier_inv_body(const HierArgs &a, long row0, int 
     }
     if (valid) {
 #pragma unroll
-      for (int s = 0; s < 16; ++s) bstore_nt(ry, u[s], vy, (uint32_t)(64 * s) * 8u);
+      for (int s = 0; s < 16; ++s) iy.st(u[s], (uint32_t)(64 * s));
     }
   }
 }

@@ -56,11 +56,14 @@ struct LineFFT {
   int fastconv(const float2 *x, long ldx, const float2 *w, long ldw, float2 *y, long ldy, long lines, float scale,
                hipStream_t s) const;
   // fastconv with X and Y in split storage (line strides in floats) and an
-  // interleaved spectrum, read and written by the kernel itself; only the
-  // hierarchical path (native_split())
+  // interleaved spectrum, and the plain transform on split lines, read and
+  // written by the kernels themselves; only the hierarchical path
+  // (native_split())
   bool native_split() const { return kind == HIER; }
   int fastconv_split(const float *xr, const float *xi, long ldx, const float2 *w, long ldw, float *yr, float *yi,
                      long ldy, long lines, float scale, hipStream_t s) const;
+  int transform_split(int dir, const float *in_re, const float *in_im, long ld_in, float *out_re, float *out_im,
+                      long ld_out, long lines, float scale, hipStream_t s) const;
   int hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *w, long ldw, long ldy, long lines,
                     float scale, hipStream_t s) const;
   int fs_fastconv(const float2 *x, long ldx, const float2 *wprep, long ldwp, float2 *y, long ldy, long lines,
