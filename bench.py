@@ -77,28 +77,41 @@ def cpu_baseline(x, w, seconds):
     oracle.fastconv_port(x[:2], w)  # warm the code path
 
     def median_pass(threads, budget):
-        times = []
-        t_end = time.perf_counter() + budget
-        while time.perf_counter() < t_end or len(times) < 5:
+        # the reference's calibration (benchmarks/loop.hpp:267-357): grow the
+        # loop count until one sample takes >= 1 s (or the budget / 5 when
+        # that is shorter), then take the median of >= 5 samples
+        goal = min(1.0, budget / 5)
+        loops = 1
+        while True:
             t0 = time.perf_counter()
-            oracle.fastconv_port(x, w, threads=threads)
-            times.append(time.perf_counter() - t0)
-        return float(np.median(times)), len(times)
+            for _ in range(loops):
+                oracle.fastconv_port(x, w, threads=threads)
+            t = time.perf_counter() - t0
+            if t >= goal:
+                break
+            loops = max(loops + 1, int(loops * 1.05 * goal / max(t, 1e-3)))
+        samples = [t / loops]
+        while len(samples) < 5:
+            t0 = time.perf_counter()
+            for _ in range(loops):
+                oracle.fastconv_port(x, w, threads=threads)
+            samples.append((time.perf_counter() - t0) / loops)
+        return float(np.median(samples)), len(samples), loops
 
-    med, count = median_pass(1, seconds)
+    med, count, loops = median_pass(1, seconds)
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
-    med_all, count_all = median_pass(cores, max(1.0, seconds / 4))
+    med_all, count_all, loops_all = median_pass(cores, max(1.0, seconds / 2))
     return {
         "value": rows / med,
         "unit": "rows/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{count} passes of the full {rows}x{x.shape[1]} matrix (median), "
+        "sample": f"median of {count} samples of {loops} passes each over the full {rows}x{x.shape[1]} matrix, "
                   "four-pass FP32 CPU fastconv (FFT rows, vmmul, inverse FFT rows, 1/N pass), 1 thread",
         "cpu_model": _cpu_model(),
         "all_cores": {"value": rows / med_all, "unit": "rows/s", "cores": cores,
-                      "sample": f"{count_all} passes (median), rows split over {cores} threads; "
-                                "not the reference's behaviour"},
+                      "sample": f"median of {count_all} samples of {loops_all} passes, rows split over "
+                                f"{cores} threads; not the reference's behaviour"},
     }
 
 
