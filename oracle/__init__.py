@@ -2,8 +2,8 @@
 
 This package restates the reference's CPU algorithms (see ovx_oracle.c for the
 file:line each function follows).  Only tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py may import it, and only as the checker or as the
-timed CPU baseline.  The product (openvsip_amd, libovxhip.so) never uses it.
+cpu_baseline leg of bench.py may import it, and only as the values GPU results
+are compared with or as the timed CPU baseline.  The product (openvsip_amd, libovxhip.so) never uses it.
 
 Pinning status of each function is listed in the header of ovx_oracle.c; every
 function has at least one independent check in tests/test_oracle.py.

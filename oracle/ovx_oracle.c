@@ -2,9 +2,9 @@
  * ovx_oracle.c -- CPU restatement of OpenVSIP's fast-convolution path.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (openvsip_amd/, the HIP
- * library, the C-ABI) links, loads or calls this file.  It is used by the
- * tests as the checker, by __graft_entry__.smoke() as the checker, and by
- * bench.py's cpu_baseline leg (the FP32 "port" of the reference CPU path).
+ * library, the C-ABI) links, loads or calls this file.  The tests and
+ * __graft_entry__.smoke() compare GPU results against it, and bench.py's
+ * cpu_baseline leg times it (the FP32 "port" of the reference CPU path).
  *
  * Every function restates one piece of the reference and cites the
  * reference file:line it follows (paths relative to the reference tree).
@@ -235,7 +235,7 @@ void ovo_fastconv_ref_dft(const float *x, long ldx, const float *w, long ldw,
 /* ------------------------------------------------------------------ */
 /* FP32 "port" of the reference CPU fastconv (the CPU baseline).       */
 /* Four full-matrix passes exactly as the reference runs them          */
-/* (SURVEY call stack B): per-row forward FFT                          */
+/* in the reference's CPU fastconv: per-row forward FFT               */
 /* (src/ovxx/fftw/fftw.cpp:965-977, one execute per row, out of place),*/
 /* the vmmul row loop (src/vsip/impl/vmmul.hpp:79-85), per-row inverse */
 /* FFT, then the separate *= 1/N pass (workspace.hpp:474-478).         */

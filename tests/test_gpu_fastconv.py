@@ -1,7 +1,7 @@
 """GPU parity tests: the HIP kernels (through the C-ABI) against the CPU
 oracle on the same seeded inputs.  Tolerance for complex<float>: rel-L2 <= 1e-5
-against the FP64 oracle (the bar BASELINE states for the reference's CPU
-fastconv), plus the reference's own test::diff < -100 dB criterion
+against the FP64 oracle (the tolerance set for complex<float> results of
+the reference's CPU fastconv), plus the reference's own test::diff < -100 dB criterion
 (support/test/diff.hpp:62-96, tests/fastconv.cpp:327)."""
 import numpy as np
 import pytest

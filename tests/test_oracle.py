@@ -1,5 +1,5 @@
-"""Pins the CPU oracle (oracle/ovx_oracle.c) before it is trusted as the
-checker.  Each check is independent of the formula it checks: golden vectors
+"""Pins the CPU oracle (oracle/ovx_oracle.c) before GPU results are compared
+with it.  Each check is independent of the formula it checks: golden vectors
 from the reference's own tests, brute force, library routines (numpy.fft /
 numpy.convolve), closed forms and invariants.  A planted mistake in an oracle
 function fails at least one test here (see test_planted_mistakes_are_caught).
