@@ -7,6 +7,8 @@
 //   op::fft   list  src/ovxx/signal/fft.hpp:35-49
 //   op::fftm  list  src/ovxx/signal/fft.hpp:51-64
 //   assign<2> list  src/ovxx/assign_fwd.hpp:49-62
+//   op::corr  list  src/vsip/impl/signal/corr.hpp:26-38
+//   op::conv  list  src/vsip/impl/signal/conv.hpp:27-38
 // so existing vsip::Fft / vsip::Fftm / vmmul code selects the HIP kernels
 // without source changes; rt_valid() returns false for a shape the backend
 // does not handle, and the dispatcher falls through to the next backend
@@ -36,6 +38,8 @@
 #include <ovxx/signal/fft.hpp>
 #include <ovxx/signal/fft/backend.hpp>
 #include <ovxx/signal/fft/functor.hpp>
+#include <ovxx/signal/conv.hpp>
+#include <ovxx/signal/corr.hpp>
 
 #include "ovxhip.hpp"
 
@@ -234,6 +238,121 @@ void fastconv_call(ovxhip::Fastconv &fc, std::pair<PX *, PX *> x, stride_type ld
   fc(ovxhip::const_split_ptr(x.first, x.second), ldx, ovxhip::const_split_ptr(w.first, w.second), 0, y, ldy, rows);
 }
 
+// ---- Correlation and Convolution (src/vsip/impl/signal/corr.hpp:31-38,
+// conv.hpp:33-38).  These dispatch at compile time: the first backend whose
+// Evaluator is ct_valid becomes the base class of vsip::Correlation /
+// vsip::Convolution (src/ovxx/dispatcher/dispatcher.hpp:89-93), so there is
+// no run-time fallthrough; a shape the library declines is run by the
+// reference's generic backend, held alongside.  Data go through DDA as dense
+// interleaved host arrays (converted by DDA from split storage).
+inline int support_of(vsip::support_region_type r)
+{
+  return r == vsip::support_full ? OVX_SUPPORT_FULL : r == vsip::support_same ? OVX_SUPPORT_SAME : OVX_SUPPORT_MIN;
+}
+inline int symmetry_of(vsip::symmetry_type s)
+{
+  return s == vsip::nonsym ? OVX_NONSYM : s == vsip::sym_even_len_odd ? OVX_SYM_EVEN_LEN_ODD : OVX_SYM_EVEN_LEN_EVEN;
+}
+typedef vsip::Layout<1, vsip::row1_type, vsip::dense, vsip::interleaved_complex> dense_interleaved;
+
+// signal::Correlation's interface (src/ovxx/signal/corr.hpp:611-661).
+template <vsip::support_region_type R, unsigned N, vsip::alg_hint_type H>
+class corr
+{
+  typedef signal::Correlation<1, R, ctype, N, H> generic_type;
+
+public:
+  static vsip::support_region_type const supprt = R;
+  corr(vsip::Domain<1> const &ref_size, vsip::Domain<1> const &input_size)
+    : ref_size_(ref_size.size()), input_size_(input_size.size()),
+      output_size_(ovx_corr_output_size(support_of(R), ref_size.size(), input_size.size()))
+  {
+    try {
+      plan_.reset(new ovxhip::Correlation(support_of(R), ref_size.size(), input_size.size(),
+                                          OVX_FLAG_HOST_POINTERS));
+    } catch (ovxhip::unimplemented const &) {
+      generic_.reset(new generic_type(ref_size, input_size));
+    }
+  }
+  vsip::Domain<1> const &reference_size() const { return ref_size_; }
+  vsip::Domain<1> const &input_size() const { return input_size_; }
+  vsip::Domain<1> const &output_size() const { return output_size_; }
+
+  template <typename B1, typename B2, typename B3>
+  void correlate(vsip::bias_type bias, vsip::const_Vector<ctype, B1> ref, vsip::const_Vector<ctype, B2> in,
+                 vsip::Vector<ctype, B3> out)
+  {
+    if (generic_) {
+      generic_->correlate(bias, ref, in, out);
+      return;
+    }
+    dda::Data<B1, dda::in, dense_interleaved> r(ref.block());
+    dda::Data<B2, dda::in, dense_interleaved> x(in.block());
+    dda::Data<B3, dda::out, dense_interleaved> y(out.block());
+    (*plan_)(bias == vsip::unbiased ? OVX_UNBIASED : OVX_BIASED,
+             reinterpret_cast<ovxhip::cscalar const *>(r.ptr()), reinterpret_cast<ovxhip::cscalar const *>(x.ptr()),
+             reinterpret_cast<ovxhip::cscalar *>(y.ptr()));
+  }
+
+private:
+  vsip::Domain<1> ref_size_, input_size_, output_size_;
+  std::unique_ptr<ovxhip::Correlation> plan_;
+  std::unique_ptr<generic_type> generic_;
+};
+
+// signal::Convolution's interface (src/ovxx/signal/conv.hpp:964-1010).
+template <vsip::symmetry_type S, vsip::support_region_type R, unsigned N, vsip::alg_hint_type H>
+class conv
+{
+  typedef signal::Convolution<vsip::const_Vector, S, R, ctype, N, H> generic_type;
+
+public:
+  static vsip::symmetry_type const symmtry = S;
+  static vsip::support_region_type const supprt = R;
+  template <typename B>
+  conv(vsip::const_Vector<ctype, B> coeffs, vsip::Domain<1> const &input_size, length_type decimation)
+    : input_size_(input_size.size()), decimation_(decimation)
+  {
+    const length_type m = S == vsip::nonsym ? coeffs.size()
+                          : S == vsip::sym_even_len_odd ? 2 * coeffs.size() - 1 : 2 * coeffs.size();
+    kernel_size_ = vsip::Domain<1>(m);
+    output_size_ = vsip::Domain<1>(ovx_conv_output_size(support_of(R), m, input_size.size(), decimation));
+    dda::Data<B, dda::in, dense_interleaved> c(coeffs.block());
+    try {
+      plan_.reset(new ovxhip::Convolution(symmetry_of(S), support_of(R),
+                                          reinterpret_cast<ovxhip::cscalar const *>(c.ptr()), coeffs.size(),
+                                          input_size.size(), decimation, OVX_FLAG_HOST_POINTERS));
+    } catch (ovxhip::unimplemented const &) {
+      generic_.reset(new generic_type(coeffs, input_size, decimation));
+    }
+  }
+  vsip::Domain<1> const &kernel_size() const { return kernel_size_; }
+  vsip::Domain<1> const &filter_order() const { return kernel_size_; }
+  vsip::Domain<1> const &input_size() const { return input_size_; }
+  vsip::Domain<1> const &output_size() const { return output_size_; }
+  vsip::symmetry_type symmetry() const { return S; }
+  vsip::support_region_type support() const { return R; }
+  length_type decimation() const { return decimation_; }
+
+  template <typename B1, typename B2>
+  void convolve(vsip::const_Vector<ctype, B1> in, vsip::Vector<ctype, B2> out)
+  {
+    if (generic_) {
+      generic_->convolve(in, out);
+      return;
+    }
+    dda::Data<B1, dda::in, dense_interleaved> x(in.block());
+    dda::Data<B2, dda::out, dense_interleaved> y(out.block());
+    (*plan_)(reinterpret_cast<ovxhip::cscalar const *>(x.ptr()), reinterpret_cast<ovxhip::cscalar *>(y.ptr()));
+  }
+
+private:
+  vsip::Domain<1> kernel_size_, input_size_, output_size_;
+  length_type decimation_;
+  std::unique_ptr<ovxhip::Convolution> plan_;
+  std::unique_ptr<generic_type> generic_;
+};
+
 } // namespace ovxx::ovxhip_backend
 
 namespace dispatcher
@@ -357,6 +476,21 @@ struct Evaluator<op::assign<2>, be::user,
     ovxhip_backend::fastconv_call(fc, m_data.ptr(), m_data.stride(0), w_data.ptr(), out_data.ptr(),
                                   out_data.stride(0), rows);
   }
+};
+
+// complex<float> 1-D correlation and convolution on the gfx950 kernels
+template <vsip::support_region_type R, unsigned N, vsip::alg_hint_type H>
+struct Evaluator<op::corr<1, R, complex<float>, N, H>, be::user>
+{
+  static bool const ct_valid = true;
+  typedef ovxhip_backend::corr<R, N, H> backend_type;
+};
+
+template <vsip::symmetry_type S, vsip::support_region_type R, unsigned N, vsip::alg_hint_type H>
+struct Evaluator<op::conv<1, S, R, complex<float>, N, H>, be::user>
+{
+  static bool const ct_valid = true;
+  typedef ovxhip_backend::conv<S, R, N, H> backend_type;
 };
 
 } // namespace ovxx::dispatcher
