@@ -67,9 +67,10 @@ def _cpu_model():
 
 
 def cpu_baseline(x, w, seconds):
-    """Single-core FP32 four-pass fastconv over the same matrix, repeated
-    until `seconds` of CPU work; median pass time -> rows/s.  Also the same
-    code with the rows split over the host's cores (not the reference's
+    """Single-core FP32 four-pass fastconv over the same matrix: the median
+    of five samples, each a loop of passes calibrated to about one second
+    (bounded by `seconds` of CPU work in all) -> rows/s.  Also the same code
+    with the rows split over the host's cores (not the reference's
     behaviour: its fftm backend loops rows on one thread)."""
     import numpy as np
     import oracle
@@ -231,8 +232,8 @@ def drop_in_section(args, x, w):
                                         "d2h_bytes_total": d2h, "calls": calls},
         "y_bitwise_equal_across_paths": same,
         "split_storage_device": {"what": "ovx_fastconv_split_exec on device arrays (real and imaginary parts "
-                                         "apart): the fused kernel reads and writes the split arrays; the "
-                                         "spectrum is interleaved and permuted per call",
+                                         "apart): the fused kernel reads and writes the split arrays; one "
+                                         "small kernel permutes and scales the split spectrum per call",
                                  "ms_per_call": split_ms, "rows_per_s": R / (split_ms * 1e-3),
                                  "bitwise_equal_to_interleaved": split_same},
     }

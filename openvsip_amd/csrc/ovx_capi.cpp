@@ -318,6 +318,36 @@ int LineFFT::transform_split(int dir, const float *in_re, const float *in_im, lo
   return e == hipSuccess ? OVX_OK : hip_fail(e, "row fft launch");
 }
 
+// Split X, Y and a single split spectrum vector: one kernel permutes and
+// scales the spectrum straight from its two arrays, then the split fastconv.
+int LineFFT::fastconv_split_vec(const float *xr, const float *xi, long ldx, const float *wr, const float *wi,
+                                float *yr, float *yi, long ldy, long lines, float scale, hipStream_t s) const
+{
+  if (lines <= 0) return OVX_OK;
+  if (kind != HIER) return fail(OVX_ERR_UNSUPPORTED, "split-storage fastconv runs natively on the hierarchical path");
+  if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
+  if (!offsets_fit(hier_rows_per_block(n), std::max(ldx, ldy)))
+    return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
+  int st = wp.alloc((size_t)n * 8);
+  if (st) return st;
+  hipError_t e = launch_hier_permute_split(wr, wi, (float2 *)wp.p, n, scale, s);
+  if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
+  HierArgs a = hier_args(*this);
+  a.xr = xr;
+  a.xi = xi;
+  a.ldx = ldx;
+  a.yr = yr;
+  a.yi = yi;
+  a.ldy = ldy;
+  a.rows = lines;
+  a.w = (const float2 *)wp.p;
+  a.ldw = 0;
+  a.natural = 0;
+  a.scale = 1.0f;
+  e = launch_hier(HIER_FASTCONV_SPLIT, n, a, s);
+  return e == hipSuccess ? OVX_OK : hip_fail(e, "fastconv launch");
+}
+
 int LineFFT::fastconv_split(const float *xr, const float *xi, long ldx, const float2 *w, long ldw, float *yr,
                             float *yi, long ldy, long lines, float scale, hipStream_t s) const
 {
@@ -935,9 +965,11 @@ extern "C" int ovx_fastconv_split_exec(ovx_plan p, const void *x_re, const void 
   hipStream_t s = (hipStream_t)stream;
   const long wlines = ldw ? rows : 1;
   auto run = [&](SplitIn x, SplitIn w, SplitOut y) -> int {
+    if (p->line.native_split() && ldw == 0)
+      return p->line.fastconv_split_vec(x.re, x.im, ldx, w.re, w.im, y.re, y.im, ldy, rows, p->scale, s);
     if (p->line.native_split()) {
       // the kernel reads and writes the split arrays itself; only the
-      // spectrum is interleaved first
+      // coefficient matrix is interleaved first
       int r = p->split_ws.alloc((size_t)wlines * n * 8);
       if (r) return r;
       float2 *ww = (float2 *)p->split_ws.p;

@@ -64,6 +64,17 @@ __global__ void k_hier_permute(const float2 *__restrict__ w, long ldw, float2 *_
   wp[row * n + i] = cscale(w[row * ldw + k1 + r0 * k2], scale);
 }
 
+// The same permutation read from a split-storage spectrum vector:
+// wp[k1*1024 + k2] = scale * (wr + i wi)[k1 + R0*k2].
+__global__ void k_hier_permute_split(const float *__restrict__ wr, const float *__restrict__ wi,
+                                     float2 *__restrict__ wp, int n, int r0, float scale)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int src = (i >> 10) + r0 * (i & 1023);
+  wp[i] = make_float2(scale * wr[src], scale * wi[src]);
+}
+
 // Outer twiddles: two[(k1-1)*1024 + j] = exp(-2 pi i j k1 / N), FP64-evaluated.
 __global__ void k_hier_outer_twiddles(float2 *two, int n, int count)
 {
@@ -123,6 +134,14 @@ hipError_t launch_hier_permute(const float2 *w, long ldw, float2 *wp, long n, lo
 {
   dim3 grid((unsigned)((n + 255) / 256), (unsigned)rows);
   hipLaunchKernelGGL(k_hier_permute, grid, dim3(256), 0, s, w, ldw, wp, (int)n, (int)(n / 1024), scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_hier_permute_split(const float *wr, const float *wi, float2 *wp, long n, float scale,
+                                     hipStream_t s)
+{
+  hipLaunchKernelGGL(k_hier_permute_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, wr, wi, wp, (int)n,
+                     (int)(n / 1024), scale);
   return hipGetLastError();
 }
 

@@ -31,6 +31,9 @@ int hier_wave_twiddle_count();
 hipError_t launch_hier(int kind, long n, const HierArgs &a, hipStream_t s);
 hipError_t launch_hier_permute(const float2 *w, long ldw, float2 *wp, long n, long rows, float scale,
                                hipStream_t s);
+// one split-storage spectrum vector (n values) into the same order
+hipError_t launch_hier_permute_split(const float *wr, const float *wi, float2 *wp, long n, float scale,
+                                     hipStream_t s);
 hipError_t launch_hier_twiddles(long n, float2 *two, float2 *tw_wave, hipStream_t s);
 
 } // namespace ovx

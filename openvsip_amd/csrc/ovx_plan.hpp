@@ -64,6 +64,8 @@ struct LineFFT {
                      long ldy, long lines, float scale, hipStream_t s) const;
   int transform_split(int dir, const float *in_re, const float *in_im, long ld_in, float *out_re, float *out_im,
                       long ld_out, long lines, float scale, hipStream_t s) const;
+  int fastconv_split_vec(const float *xr, const float *xi, long ldx, const float *wr, const float *wi, float *yr,
+                         float *yi, long ldy, long lines, float scale, hipStream_t s) const;
   int hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *w, long ldw, long ldy, long lines,
                     float scale, hipStream_t s) const;
   int fs_fastconv(const float2 *x, long ldx, const float2 *wprep, long ldwp, float2 *y, long ldy, long lines,
