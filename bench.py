@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--serve-rates", default="2000,5000,10000,20000", help="offered rates, jobs/s")
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="arrival window per rate")
     ap.add_argument("--serve-saturation-jobs", type=int, default=3000)
+    ap.add_argument("--serve-slots", type=int, default=3, help="batches in flight in the batched server")
     return ap.parse_args()
 
 
@@ -412,9 +413,10 @@ def serve_section(args):
 
     out = {"workload": "Poisson arrivals; 80% fastconv (N in {512..16384,65536,3072,6144,12288}, rows U[16,256]), "
                        "10% forward Fftm (same sampling), 10% correlation (N=65536, M=1024, full); 64 resident "
-                       "replicas; HBM budget 64 GB; 3 batches in flight",
+                       f"replicas; HBM budget 64 GB; {args.serve_slots} batches in flight",
            "rates": {}}
-    servers = {"batched": ovx.Server(slots=3, max_jobs=512, hbm_budget_mb=64 * 1024, workspace_mb=2048),
+    servers = {"batched": ovx.Server(slots=args.serve_slots, max_jobs=512, hbm_budget_mb=64 * 1024,
+                                     workspace_mb=2048),
                "plain": ovx.Server(slots=1, plain=True, hbm_budget_mb=64 * 1024, workspace_mb=2048),
                "plain_3_streams": ovx.Server(slots=3, plain=True, hbm_budget_mb=64 * 1024, workspace_mb=2048)}
     for srv in servers.values():
