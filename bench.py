@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="arrival window per rate")
     ap.add_argument("--serve-saturation-jobs", type=int, default=3000)
     ap.add_argument("--serve-slots", type=int, default=3, help="batches in flight in the batched server")
+    ap.add_argument("--serve-max-jobs", type=int, default=512, help="requests per batch in the batched server")
+    ap.add_argument("--serve-rounds", type=int, default=5,
+                    help="saturation rounds (a fresh trace each, run by every path; medians reported)")
     return ap.parse_args()
 
 
@@ -357,7 +360,7 @@ def serve_section(args):
     (N, rows sampled as in the batched workload), 10% forward Fftm (same
     sampling) and 10% correlation (input 65536, reference 1024, full
     support); 64 distinct replicas resident; admission by a 64 GB HBM budget.
-    The continuous-batching server (3 slots in flight) against the plain
+    The continuous-batching server (3 slots in flight by default) against the plain
     per-request path (one standalone launch per request, launched when the
     previous request has completed) and the same launches with 3 requests in
     flight on separate streams.  Requests reuse a
@@ -415,19 +418,26 @@ def serve_section(args):
                        "10% forward Fftm (same sampling), 10% correlation (N=65536, M=1024, full); 64 resident "
                        f"replicas; HBM budget 64 GB; {args.serve_slots} batches in flight",
            "rates": {}}
-    servers = {"batched": ovx.Server(slots=args.serve_slots, max_jobs=512, hbm_budget_mb=64 * 1024,
+    servers = {"batched": ovx.Server(slots=args.serve_slots, max_jobs=args.serve_max_jobs, hbm_budget_mb=64 * 1024,
                                      workspace_mb=2048),
                "plain": ovx.Server(slots=1, plain=True, hbm_budget_mb=64 * 1024, workspace_mb=2048),
                "plain_3_streams": ovx.Server(slots=3, plain=True, hbm_budget_mb=64 * 1024, workspace_mb=2048)}
     for srv in servers.values():
         srv.run_trace(make_trace(200, 0))  # warm every size's path and tables
-    for name, srv in servers.items():
+    # saturation: every path runs the same trace of jobs, all arriving at
+    # once; several rounds with fresh traces, medians reported
+    rates = {name: [] for name in servers}
+    for _ in range(args.serve_rounds):
         reqs = make_trace(args.serve_saturation_jobs, 0)
-        lat, mk = srv.run_trace(reqs)
-        out[name + "_saturation_jobs_per_s"] = len(reqs) / mk
-    out["saturation_speedup"] = out["batched_saturation_jobs_per_s"] / out["plain_saturation_jobs_per_s"]
-    out["saturation_speedup_vs_plain_3_streams"] = (out["batched_saturation_jobs_per_s"]
-                                                    / out["plain_3_streams_saturation_jobs_per_s"])
+        for name, srv in servers.items():
+            lat, mk = srv.run_trace(reqs)
+            rates[name].append(len(reqs) / mk)
+    for name in servers:
+        out[name + "_saturation_jobs_per_s"] = float(np.median(rates[name]))
+    out["saturation_speedup"] = float(np.median(np.array(rates["batched"]) / np.array(rates["plain"])))
+    out["saturation_speedup_vs_plain_3_streams"] = float(
+        np.median(np.array(rates["batched"]) / np.array(rates["plain_3_streams"])))
+    out["saturation_rounds"] = args.serve_rounds
     for rate in [float(r) for r in args.serve_rates.split(",") if r]:
         row = {}
         count = max(50, int(rate * args.serve_seconds))
