@@ -301,8 +301,15 @@ int LineFFT::transform_split(int dir, const float *in_re, const float *in_im, lo
                              float *out_im, long ld_out, long lines, float scale, hipStream_t s) const
 {
   if (lines <= 0) return OVX_OK;
-  if (kind != HIER) return fail(OVX_ERR_UNSUPPORTED, "split-storage transforms run natively on the hierarchical path");
+  if (!native_split()) return fail(OVX_ERR_UNSUPPORTED, "no split-storage transform kernel on this path");
   if (!tables_ready()) return fail(OVX_ERR_STATE, "transform plan has no twiddle tables");
+  if (kind == LDS) {
+    if ((unsigned long)rc.rows_per_block * (unsigned long)std::max(ld_in, ld_out) * 8ul >= (1ul << 32))
+      return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
+    hipError_t e = launch_row_fft_split(rc, dir, in_re, in_im, ld_in, out_re, out_im, ld_out, lines,
+                                        (const float2 *)tw.p, scale, s);
+    return e == hipSuccess ? OVX_OK : hip_fail(e, "row fft launch");
+  }
   if (!offsets_fit(hier_rows_per_block(n), std::max(ld_in, ld_out)))
     return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
   HierArgs a = hier_args(*this);
@@ -324,12 +331,18 @@ int LineFFT::fastconv_split_vec(const float *xr, const float *xi, long ldx, cons
                                 float *yr, float *yi, long ldy, long lines, float scale, hipStream_t s) const
 {
   if (lines <= 0) return OVX_OK;
-  if (kind != HIER) return fail(OVX_ERR_UNSUPPORTED, "split-storage fastconv runs natively on the hierarchical path");
+  if (!native_split()) return fail(OVX_ERR_UNSUPPORTED, "no split-storage fastconv kernel on this path");
   if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
-  if (!offsets_fit(hier_rows_per_block(n), std::max(ldx, ldy)))
-    return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
   int st = wp.alloc((size_t)n * 8);
   if (st) return st;
+  if (kind == LDS) {
+    // the register-Stockham kernel reads the spectrum in natural order
+    hipError_t e = launch_interleave(wr, wi, 0, (float2 *)wp.p, n, 1, n, s);
+    if (e != hipSuccess) return hip_fail(e, "interleave");
+    return fastconv_split(xr, xi, ldx, (const float2 *)wp.p, 0, yr, yi, ldy, lines, scale, s);
+  }
+  if (!offsets_fit(hier_rows_per_block(n), std::max(ldx, ldy)))
+    return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
   hipError_t e = launch_hier_permute_split(wr, wi, (float2 *)wp.p, n, scale, s);
   if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
   HierArgs a = hier_args(*this);
@@ -352,8 +365,16 @@ int LineFFT::fastconv_split(const float *xr, const float *xi, long ldx, const fl
                             float *yi, long ldy, long lines, float scale, hipStream_t s) const
 {
   if (lines <= 0) return OVX_OK;
-  if (kind != HIER) return fail(OVX_ERR_UNSUPPORTED, "split-storage fastconv runs natively on the hierarchical path");
+  if (!native_split()) return fail(OVX_ERR_UNSUPPORTED, "no split-storage fastconv kernel on this path");
   if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
+  if (kind == LDS) {
+    long ld = std::max(std::max(ldx, ldy), ldw);
+    if ((unsigned long)rc.rows_per_block * (unsigned long)ld * 8ul >= (1ul << 32))
+      return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
+    hipError_t e = launch_row_fastconv_split(rc, xr, xi, ldx, w, ldw, yr, yi, ldy, lines, (const float2 *)tw.p,
+                                             scale, s);
+    return e == hipSuccess ? OVX_OK : hip_fail(e, "fastconv launch");
+  }
   HierArgs a = hier_args(*this);
   a.xr = xr;
   a.xi = xi;
@@ -852,10 +873,10 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
 // Split-complex storage (OpenVSIP's split_complex format: real and imaginary
 // parts in two float arrays with the same strides), the fftm_backend
 // overloads on std::pair<T*, T*> (backend.hpp:741-783; the FFTW backend's
-// split DFTs, fftw.cpp:945-1001).  The hierarchical kernels (fastconv and
-// transforms) read and write split arrays themselves (two 32-bit accesses
-// per value instead of one 64-bit one).  Everything else goes through the
-// plan's workspace: the
+// split DFTs, fftw.cpp:945-1001).  The hierarchical and register-Stockham
+// kernels (fastconv and transforms) read and write split arrays themselves
+// (two 32-bit accesses per value instead of one 64-bit one).  The four-step
+// path goes through the plan's workspace: the
 // lines are interleaved, the interleaved path runs there in place, and the
 // result is split into the output arrays -- two extra passes over the data.
 // Either way the floats equal the interleaved call's on the same values.

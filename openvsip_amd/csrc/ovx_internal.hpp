@@ -40,6 +40,15 @@ hipError_t launch_row_fastconv(const RowConfig &cfg, const float2 *x, long ldx,
                                const float2 *w, long ldw, float2 *y, long ldy, long rows,
                                const float2 *tw, float scale, hipStream_t s);
 
+// The same two on split storage: real and imaginary parts of the lines in
+// two float arrays with the same strides (ld in floats); w stays interleaved.
+hipError_t launch_row_fft_split(const RowConfig &cfg, int dir, const float *in_re, const float *in_im, long ld_in,
+                                float *out_re, float *out_im, long ld_out, long rows, const float2 *tw, float scale,
+                                hipStream_t s);
+hipError_t launch_row_fastconv_split(const RowConfig &cfg, const float *xr, const float *xi, long ldx,
+                                     const float2 *w, long ldw, float *yr, float *yi, long ldy, long rows,
+                                     const float2 *tw, float scale, hipStream_t s);
+
 // Forward twiddle table of a radix plan: tw[twoff_s + (r-1)*Ns + k] =
 // exp(-2 pi i r k / (Ns R_s)) evaluated in FP64 on the device, rounded to FP32.
 hipError_t launch_twiddles(const RadixPlan &plan, float2 *tw, hipStream_t s);

@@ -57,9 +57,9 @@ struct LineFFT {
                hipStream_t s) const;
   // fastconv with X and Y in split storage (line strides in floats) and an
   // interleaved spectrum, and the plain transform on split lines, read and
-  // written by the kernels themselves; only the hierarchical path
-  // (native_split())
-  bool native_split() const { return kind == HIER; }
+  // written by the kernels themselves; the hierarchical and register-Stockham
+  // paths (native_split())
+  bool native_split() const { return kind == HIER || kind == LDS; }
   int fastconv_split(const float *xr, const float *xi, long ldx, const float2 *w, long ldw, float *yr, float *yi,
                      long ldy, long lines, float scale, hipStream_t s) const;
   int transform_split(int dir, const float *in_re, const float *in_im, long ld_in, float *out_re, float *out_im,

@@ -49,6 +49,47 @@ __global__ __launch_bounds__(N / P * B) void k_row_fastconv(const float2 *__rest
                           lds_all + lrow * E::LROW);
 }
 
+// Split-storage forms: real and imaginary parts of the lines in two float
+// arrays with the same strides (the spectrum stays interleaved).
+template <int N, int P, int B, int DIR>
+__global__ __launch_bounds__(N / P * B) void k_row_fft_split(const float *__restrict__ in_re,
+                                                            const float *__restrict__ in_im, long ld_in,
+                                                            float *__restrict__ out_re, float *__restrict__ out_im,
+                                                            long ld_out, long rows, const float2 *__restrict__ tw,
+                                                            float scale)
+{
+  using E = Engine<N, P, (N / P <= 64)>;
+  __shared__ float2 lds_all[B * E::LROW];
+  const int lrow = (B == 1) ? 0 : (int)threadIdx.x / E::T;
+  const int t = (B == 1) ? (int)threadIdx.x : (int)threadIdx.x % E::T;
+  const long row0 = (long)blockIdx.x * B;
+  row_fft_io<N, P, DIR>(IoS{make_rsrc(in_re + row0 * ld_in, 0xffffffffu), make_rsrc(in_im + row0 * ld_in, 0xffffffffu),
+                            (uint32_t)(lrow * ld_in + t) * 4u},
+                        IoS{make_rsrc(out_re + row0 * ld_out, 0xffffffffu),
+                            make_rsrc(out_im + row0 * ld_out, 0xffffffffu), (uint32_t)(lrow * ld_out + t) * 4u},
+                        rows, tw, scale, row0, lrow, t, lds_all + lrow * E::LROW);
+}
+
+template <int N, int P, int B>
+__global__ __launch_bounds__(N / P * B) void k_row_fastconv_split(const float *__restrict__ xr,
+                                                                 const float *__restrict__ xi, long ldx,
+                                                                 const float2 *__restrict__ w, long ldw,
+                                                                 float *__restrict__ yr, float *__restrict__ yi,
+                                                                 long ldy, long rows, const float2 *__restrict__ tw,
+                                                                 float scale)
+{
+  using E = Engine<N, P, (N / P <= 64)>;
+  __shared__ float2 lds_all[B * E::LROW];
+  const int lrow = (B == 1) ? 0 : (int)threadIdx.x / E::T;
+  const int t = (B == 1) ? (int)threadIdx.x : (int)threadIdx.x % E::T;
+  const long row0 = (long)blockIdx.x * B;
+  row_fastconv_io<N, P>(IoS{make_rsrc(xr + row0 * ldx, 0xffffffffu), make_rsrc(xi + row0 * ldx, 0xffffffffu),
+                            (uint32_t)(lrow * ldx + t) * 4u},
+                        IoS{make_rsrc(yr + row0 * ldy, 0xffffffffu), make_rsrc(yi + row0 * ldy, 0xffffffffu),
+                            (uint32_t)(lrow * ldy + t) * 4u},
+                        w, ldw, rows, tw, scale, row0, lrow, t, lds_all + lrow * E::LROW);
+}
+
 // ---------------------------------------------------------------- twiddles
 struct TwStageDesc {
   int nst;
@@ -164,6 +205,32 @@ static hipError_t do_fastconv(const float2 *x, long ldx, const float2 *w, long l
   return hipGetLastError();
 }
 
+template <int N, int P>
+static hipError_t do_fft_split(int dir, const float *ir, const float *ii, long ld_in, float *orr, float *oi,
+                               long ld_out, long rows, const float2 *tw, float scale, hipStream_t s)
+{
+  using C = RowCfg<N, P>;
+  dim3 grid((unsigned)((rows + C::B - 1) / C::B)), block(C::NT);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_row_fft_split<N, P, C::B, -1>), grid, block, 0, s, ir, ii, ld_in, orr, oi, ld_out, rows,
+                       tw, scale);
+  else
+    hipLaunchKernelGGL((k_row_fft_split<N, P, C::B, +1>), grid, block, 0, s, ir, ii, ld_in, orr, oi, ld_out, rows,
+                       tw, scale);
+  return hipGetLastError();
+}
+
+template <int N, int P>
+static hipError_t do_fastconv_split(const float *xr, const float *xi, long ldx, const float2 *w, long ldw, float *yr,
+                                    float *yi, long ldy, long rows, const float2 *tw, float scale, hipStream_t s)
+{
+  using C = RowCfg<N, P>;
+  dim3 grid((unsigned)((rows + C::B - 1) / C::B)), block(C::NT);
+  hipLaunchKernelGGL((k_row_fastconv_split<N, P, C::B>), grid, block, 0, s, xr, xi, ldx, w, ldw, yr, yi, ldy, rows,
+                     tw, scale);
+  return hipGetLastError();
+}
+
 // X(N, P): every compiled configuration; the first listed for an N is
 // variant 0 (the default).
 #define OVX_ROW_CONFIGS(X)                                                      \
@@ -183,9 +250,14 @@ struct Entry {
   hipError_t (*fft)(int, const float2 *, long, float2 *, long, long, const float2 *, float, hipStream_t);
   hipError_t (*fconv)(const float2 *, long, const float2 *, long, float2 *, long, long, const float2 *, float,
                       hipStream_t);
+  hipError_t (*fft_split)(int, const float *, const float *, long, float *, float *, long, long, const float2 *,
+                          float, hipStream_t);
+  hipError_t (*fconv_split)(const float *, const float *, long, const float2 *, long, float *, float *, long, long,
+                            const float2 *, float, hipStream_t);
 };
 
-#define OVX_ENTRY(N, P) { N, P, &fill_cfg<N, P>, &do_fft<N, P>, &do_fastconv<N, P> },
+#define OVX_ENTRY(N, P)                                                                              \
+  { N, P, &fill_cfg<N, P>, &do_fft<N, P>, &do_fastconv<N, P>, &do_fft_split<N, P>, &do_fastconv_split<N, P> },
 static const Entry kEntries[] = { OVX_ROW_CONFIGS(OVX_ENTRY) };
 static const int kNumEntries = (int)(sizeof(kEntries) / sizeof(kEntries[0]));
 
@@ -232,6 +304,26 @@ hipError_t launch_row_fastconv(const RowConfig &cfg, const float2 *x, long ldx, 
   if (!e) return hipErrorInvalidValue;
   if (rows <= 0) return hipSuccess;
   return e->fconv(x, ldx, w, ldw, y, ldy, rows, tw, scale, s);
+}
+
+hipError_t launch_row_fft_split(const RowConfig &cfg, int dir, const float *in_re, const float *in_im, long ld_in,
+                                float *out_re, float *out_im, long ld_out, long rows, const float2 *tw, float scale,
+                                hipStream_t s)
+{
+  const Entry *e = find_entry(cfg.n, cfg.variant);
+  if (!e) return hipErrorInvalidValue;
+  if (rows <= 0) return hipSuccess;
+  return e->fft_split(dir, in_re, in_im, ld_in, out_re, out_im, ld_out, rows, tw, scale, s);
+}
+
+hipError_t launch_row_fastconv_split(const RowConfig &cfg, const float *xr, const float *xi, long ldx,
+                                     const float2 *w, long ldw, float *yr, float *yi, long ldy, long rows,
+                                     const float2 *tw, float scale, hipStream_t s)
+{
+  const Entry *e = find_entry(cfg.n, cfg.variant);
+  if (!e) return hipErrorInvalidValue;
+  if (rows <= 0) return hipSuccess;
+  return e->fconv_split(xr, xi, ldx, w, ldw, yr, yi, ldy, rows, tw, scale, s);
 }
 
 } // namespace ovx
