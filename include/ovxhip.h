@@ -201,10 +201,10 @@ int ovx_fastconv_exec_replica(ovx_plan plan, const void *x, long ldx, int replic
    split_complex format), on the plans of the interleaved entry points.
    Results equal the interleaved call's on the same values bit for bit.  For
    lines up to 16384 points the fastconv and complex transform kernels read
-   and write split arrays themselves; the four-step path (longer lines) and
-   the real transforms' complex side interleave the lines into the plan's
-   workspace, transform them there and split them back (two extra passes
-   over the data). */
+   and write split arrays themselves, as do the real transforms' passes on
+   their complex side; the four-step path (longer lines) interleaves the
+   lines into the plan's workspace, transforms them there and splits them
+   back (two extra passes over the data). */
 int ovx_fftm_split_out_of_place(ovx_plan plan, const void *in_re,
                                 const void *in_im, long in_s0, long in_s1,
                                 void *out_re, void *out_im, long out_s0,

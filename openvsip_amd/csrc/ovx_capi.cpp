@@ -1030,8 +1030,8 @@ extern "C" int ovx_fastconv_split_exec(ovx_plan p, const void *x_re, const void 
 
 // Real Fftm with a split complex side (fftm_backend<T, complex<T>> and
 // <complex<T>, T> split overloads, backend.hpp:600-694; FFTW's split r2c /
-// c2r, fftw.cpp:778-903): the complex side goes through the plan's
-// interleaved workspace (h + 1 values per line, dense).
+// c2r, fftw.cpp:778-903): the r2c post-pass writes, and the c2r pre-pass
+// reads, the split arrays themselves.
 extern "C" int ovx_fftm_r2c_split(ovx_plan p, const void *in, long in_s0, long in_s1, void *out_re, void *out_im,
                                   long out_s0, long out_s1, long rows, long cols, void *stream)
 {
@@ -1047,12 +1047,15 @@ extern "C" int ovx_fftm_r2c_split(ovx_plan p, const void *in, long in_s0, long i
   hipStream_t s = (hipStream_t)stream;
   const long hc = g.h + 1;
   auto run = [&](const void *din, SplitOut o) -> int {
+    // the h-point transform into the workspace, then the post-pass writes
+    // the split spectrum straight from it
     int r = p->split_ws.alloc((size_t)g.lines * hc * 8);
     if (r) return r;
     float2 *ws = (float2 *)p->split_ws.p;
-    if ((r = real_run(p, g, din, ws, hc, s))) return r;
-    hipError_t e = launch_deinterleave(ws, hc, o.re, o.im, g.ldc, g.lines, hc, s);
-    return e == hipSuccess ? OVX_OK : hip_fail(e, "deinterleave");
+    if ((r = p->line.transform(-1, (const float2 *)din, g.ldz, ws, hc, g.lines, 1.0f, s))) return r;
+    hipError_t e = launch_r2c_post_split(ws, hc, o.re, o.im, g.ldc, g.lines, g.h, (const float2 *)p->wtab.p,
+                                         p->scale, s);
+    return e == hipSuccess ? OVX_OK : hip_fail(e, "r2c pass");
   };
   if (!host) return run(in, {(float *)out_re, (float *)out_im});
   const HostLines gr = host_lines(g.lines, g.ldr, g.n, 4), gc = host_lines(g.lines, g.ldc, hc, 4);
@@ -1082,12 +1085,12 @@ extern "C" int ovx_fftm_c2r_split(ovx_plan p, const void *in_re, const void *in_
   hipStream_t s = (hipStream_t)stream;
   const long hc = g.h + 1;
   auto run = [&](SplitIn i, void *dout) -> int {
-    int r = p->split_ws.alloc((size_t)g.lines * hc * 8);
-    if (r) return r;
-    float2 *ws = (float2 *)p->split_ws.p;
-    hipError_t e = launch_interleave(i.re, i.im, g.ldc, ws, hc, g.lines, hc, s);
-    if (e != hipSuccess) return hip_fail(e, "interleave");
-    return real_run(p, g, ws, dout, hc, s);
+    // the pre-pass reads the split spectrum, then the inverse h-point
+    // transform in place on the output
+    hipError_t e = launch_c2r_pre_split(i.re, i.im, g.ldc, (float2 *)dout, g.ldz, g.lines, g.h,
+                                        (const float2 *)p->wtab.p, s);
+    if (e != hipSuccess) return hip_fail(e, "c2r pass");
+    return p->line.transform(+1, (const float2 *)dout, g.ldz, (float2 *)dout, g.ldz, g.lines, p->scale, s);
   };
   if (!host) return run({(const float *)in_re, (const float *)in_im}, out);
   const HostLines gr = host_lines(g.lines, g.ldr, g.n, 4), gc = host_lines(g.lines, g.ldc, hc, 4);

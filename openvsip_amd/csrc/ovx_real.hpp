@@ -12,5 +12,12 @@ hipError_t launch_r2c_post(float2 *y, long ldy, long lines, long h, const float2
 // inverse transform is the packed real signal (out of place).
 hipError_t launch_c2r_pre(const float2 *x, long ldx, float2 *z, long ldz, long lines, long h, const float2 *wt,
                           hipStream_t s);
+// The same two with the half spectra in split storage (separate real and
+// imaginary arrays, line stride in floats): the post-pass reads FFT_h from z
+// and writes X there; the pre-pass reads X from there.
+hipError_t launch_r2c_post_split(const float2 *z, long ldz, float *yr, float *yi, long ldy, long lines, long h,
+                                 const float2 *wt, float scale, hipStream_t s);
+hipError_t launch_c2r_pre_split(const float *xr, const float *xi, long ldx, float2 *z, long ldz, long lines, long h,
+                                const float2 *wt, hipStream_t s);
 
 } // namespace ovx
