@@ -199,12 +199,10 @@ int ovx_fastconv_exec_replica(ovx_plan plan, const void *x, long ldx, int replic
    131-204; the FFTW backend's split DFTs, fftw.cpp:945-1001): real and
    imaginary parts in two float arrays with the same strides (OpenVSIP's
    split_complex format), on the plans of the interleaved entry points.
-   Results equal the interleaved call's on the same values bit for bit.  For
-   lines up to 16384 points the fastconv and complex transform kernels read
-   and write split arrays themselves, as do the real transforms' passes on
-   their complex side; the four-step path (longer lines) interleaves the
-   lines into the plan's workspace, transforms them there and splits them
-   back (two extra passes over the data). */
+   Results equal the interleaved call's on the same values bit for bit.  The
+   kernels read and write split arrays themselves: the fastconv and complex
+   transform kernels of every length (the four-step path in its first and
+   last pass) and the real transforms' passes on their complex side. */
 int ovx_fftm_split_out_of_place(ovx_plan plan, const void *in_re,
                                 const void *in_im, long in_s0, long in_s1,
                                 void *out_re, void *out_im, long out_s0,

@@ -303,6 +303,30 @@ int LineFFT::transform_split(int dir, const float *in_re, const float *in_im, lo
   if (lines <= 0) return OVX_OK;
   if (!native_split()) return fail(OVX_ERR_UNSUPPORTED, "no split-storage transform kernel on this path");
   if (!tables_ready()) return fail(OVX_ERR_STATE, "transform plan has no twiddle tables");
+  if (kind == FOURSTEP) {
+    // column pass from the split input into the workspace, row pass with
+    // the transposed store into the split output
+    if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step transform");
+    int st = work.alloc((size_t)lines * n * 8);
+    if (st) return st;
+    FsArgs a = fs_args(*this);
+    a.in_re = in_re;
+    a.in_im = in_im;
+    a.ld_in = ld_in;
+    a.out = (float2 *)work.p;
+    a.ld_out = n;
+    hipError_t e = launch_fs_col(dir < 0 ? 0 : 2, a, lines, s);
+    if (e != hipSuccess) return hip_fail(e, "four-step column pass");
+    FsArgs b = fs_args(*this);
+    b.in = (const float2 *)work.p;
+    b.ld_in = n;
+    b.out_re = out_re;
+    b.out_im = out_im;
+    b.ld_out = ld_out;
+    b.scale = scale;
+    e = launch_fs_row(dir < 0 ? 1 : 2, b, lines, s);
+    return e == hipSuccess ? OVX_OK : hip_fail(e, "four-step row pass");
+  }
   if (kind == LDS) {
     if ((unsigned long)rc.rows_per_block * (unsigned long)std::max(ld_in, ld_out) * 8ul >= (1ul << 32))
       return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
@@ -333,6 +357,16 @@ int LineFFT::fastconv_split_vec(const float *xr, const float *xi, long ldx, cons
   if (lines <= 0) return OVX_OK;
   if (!native_split()) return fail(OVX_ERR_UNSUPPORTED, "no split-storage fastconv kernel on this path");
   if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
+  if (kind == FOURSTEP) {
+    // interleave the spectrum into the workspace's tail; fastconv_split
+    // permutes it before its first pass writes the head
+    int st = work.alloc((size_t)(lines + 1) * n * 8);
+    if (st) return st;
+    float2 *wi2 = (float2 *)work.p + (size_t)lines * n;
+    hipError_t e = launch_interleave(wr, wi, 0, wi2, n, 1, n, s);
+    if (e != hipSuccess) return hip_fail(e, "interleave");
+    return fastconv_split(xr, xi, ldx, wi2, 0, yr, yi, ldy, lines, scale, s);
+  }
   int st = wp.alloc((size_t)n * 8);
   if (st) return st;
   if (kind == LDS) {
@@ -367,6 +401,37 @@ int LineFFT::fastconv_split(const float *xr, const float *xi, long ldx, const fl
   if (lines <= 0) return OVX_OK;
   if (!native_split()) return fail(OVX_ERR_UNSUPPORTED, "no split-storage fastconv kernel on this path");
   if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
+  if (kind == FOURSTEP) {
+    // the three passes of fs_fastconv with X read split by the first and Y
+    // written split by the last; the middle pass works in the workspace
+    if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step fastconv");
+    const long wrows = ldw ? lines : 1;
+    int st = wp.alloc((size_t)wrows * n * 8);
+    if (st) return st;
+    hipError_t e = launch_fs_permute(w, ldw, (float2 *)wp.p, n1, n2, wrows, scale, s);
+    if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
+    if ((st = work.alloc((size_t)lines * n * 8))) return st;
+    float2 *ws = (float2 *)work.p;
+    FsArgs a = fs_args(*this);
+    a.wp = (const float2 *)wp.p;
+    a.ldwp = ldw ? n : 0;
+    a.in_re = xr;
+    a.in_im = xi;
+    a.ld_in = ldx;
+    a.out = ws;
+    a.ld_out = n;
+    if ((e = launch_fs_col(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
+    a.in_re = a.in_im = nullptr;
+    a.in = ws;
+    a.ld_in = n;
+    if ((e = launch_fs_row(0, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step row pass");
+    a.out = nullptr;
+    a.out_re = yr;
+    a.out_im = yi;
+    a.ld_out = ldy;
+    if ((e = launch_fs_col(1, a, lines, s)) != hipSuccess) return hip_fail(e, "four-step column pass");
+    return OVX_OK;
+  }
   if (kind == LDS) {
     long ld = std::max(std::max(ldx, ldy), ldw);
     if ((unsigned long)rc.rows_per_block * (unsigned long)ld * 8ul >= (1ul << 32))
@@ -873,10 +938,10 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
 // Split-complex storage (OpenVSIP's split_complex format: real and imaginary
 // parts in two float arrays with the same strides), the fftm_backend
 // overloads on std::pair<T*, T*> (backend.hpp:741-783; the FFTW backend's
-// split DFTs, fftw.cpp:945-1001).  The hierarchical and register-Stockham
-// kernels (fastconv and transforms) read and write split arrays themselves
-// (two 32-bit accesses per value instead of one 64-bit one).  The four-step
-// path goes through the plan's workspace: the
+// split DFTs, fftw.cpp:945-1001).  The kernels read and write split arrays
+// themselves (two 32-bit accesses per value instead of one 64-bit one); the
+// four-step path reads them in its first pass and writes them in its last.
+// A plan without a transform path goes through the plan's workspace: the
 // lines are interleaved, the interleaved path runs there in place, and the
 // result is split into the output arrays -- two extra passes over the data.
 // Either way the floats equal the interleaved call's on the same values.

@@ -22,18 +22,18 @@
 
 namespace ovx {
 
-template <int L, int P, int C, int MODE>
+template <int L, int P, int C, int MODE, bool SIN, bool SOUT>
 __global__ __launch_bounds__(C *(L / P)) void k_fs_col(FsArgs a)
 {
   __shared__ float2 lds_all[C * FsLds<L>::LROW];
-  fs_col_body<L, P, C, MODE>(a, blockIdx.y, blockIdx.x, threadIdx.x, lds_all);
+  fs_col_body<L, P, C, MODE, SIN, SOUT>(a, blockIdx.y, blockIdx.x, threadIdx.x, lds_all);
 }
 
-template <int L, int P, int C, int MODE>
+template <int L, int P, int C, int MODE, bool SOUT>
 __global__ __launch_bounds__(C *(L / P)) void k_fs_row(FsArgs a)
 {
   __shared__ float2 lds_all[C * FsLds<L>::LROW];
-  fs_row_body<L, P, C, MODE>(a, blockIdx.y, blockIdx.x, threadIdx.x, lds_all);
+  fs_row_body<L, P, C, MODE, SOUT>(a, blockIdx.y, blockIdx.x, threadIdx.x, lds_all);
 }
 
 // Wp[k1][k2] = scale * W[k1 + N1 k2] for `rows` coefficient rows (ldw = 0:
@@ -57,7 +57,13 @@ static hipError_t col_launch(const FsArgs &a, long rows, hipStream_t s)
 {
   using F = FsCfg<L>;
   dim3 grid((unsigned)(a.n2 / F::C), (unsigned)rows);
-  hipLaunchKernelGGL((k_fs_col<L, F::P, F::C, MODE>), grid, dim3(F::NT), 0, s, a);
+  // split storage is read by a call's first pass or written by its last
+  if (a.in_re)
+    hipLaunchKernelGGL((k_fs_col<L, F::P, F::C, MODE, true, false>), grid, dim3(F::NT), 0, s, a);
+  else if (a.out_re)
+    hipLaunchKernelGGL((k_fs_col<L, F::P, F::C, MODE, false, true>), grid, dim3(F::NT), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_fs_col<L, F::P, F::C, MODE, false, false>), grid, dim3(F::NT), 0, s, a);
   return hipGetLastError();
 }
 
@@ -66,7 +72,10 @@ static hipError_t row_launch(const FsArgs &a, long rows, hipStream_t s)
 {
   using F = FsCfg<L>;
   dim3 grid((unsigned)(a.n1 / F::C), (unsigned)rows);
-  hipLaunchKernelGGL((k_fs_row<L, F::P, F::C, MODE>), grid, dim3(F::NT), 0, s, a);
+  if (a.out_re)
+    hipLaunchKernelGGL((k_fs_row<L, F::P, F::C, MODE, true>), grid, dim3(F::NT), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_fs_row<L, F::P, F::C, MODE, false>), grid, dim3(F::NT), 0, s, a);
   return hipGetLastError();
 }
 

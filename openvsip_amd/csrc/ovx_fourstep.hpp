@@ -17,6 +17,10 @@ struct FsArgs {
   const float2 *wp;        // fastconv: permuted, scaled spectrum [k1][k2]
   long ldwp;               // 0: one spectrum for every row
   float scale;
+  // split storage for a call's first read / last write (separate real and
+  // imaginary arrays at the strides ld_in / ld_out); null: in / out are used
+  const float *in_re = nullptr, *in_im = nullptr;
+  float *out_re = nullptr, *out_im = nullptr;
 };
 
 hipError_t launch_fs_col(int mode, const FsArgs &a, long rows, hipStream_t s);

@@ -29,7 +29,27 @@ __device__ __forceinline__ float2 big_twiddle(const float2 *__restrict__ tlo, co
 // One group of C columns of one row: `group` selects columns
 // [group*C, group*C + C), `tid` is the thread within the group's C*(L/P)
 // threads, `lds_all` the group's C*LROW LDS elements.
-template <int L, int P, int C, int MODE, typename LP>
+// The passes' global reads and writes at element offset o from a call's
+// base: interleaved lines (a.in, a.out) or, when SPLIT, separate real and
+// imaginary arrays (a.in_re / a.in_im, a.out_re / a.out_im).
+template <bool SPLIT>
+__device__ __forceinline__ float2 fs_load(const FsArgs &a, long o)
+{
+  if constexpr (SPLIT) return make_float2(a.in_re[o], a.in_im[o]);
+  else return a.in[o];
+}
+template <bool SPLIT>
+__device__ __forceinline__ void fs_store(const FsArgs &a, long o, float2 v)
+{
+  if constexpr (SPLIT) {
+    a.out_re[o] = v.x;
+    a.out_im[o] = v.y;
+  } else {
+    a.out[o] = v;
+  }
+}
+
+template <int L, int P, int C, int MODE, bool SIN = false, bool SOUT = false, typename LP>
 __device__ __forceinline__ void fs_col_body(const FsArgs &a, long row, long group, int tid, LP lds_all)
 {
   using E = Engine<L, P>;
@@ -39,14 +59,13 @@ __device__ __forceinline__ void fs_col_body(const FsArgs &a, long row, long grou
   const int t = tid / C;
   const long n2 = group * C + c;
   const long N2 = a.n2;
-  const float2 *src = a.in + row * a.ld_in + n2;
-  float2 *dst = a.out + row * a.ld_out + n2;
+  const long ib = row * a.ld_in + n2, ob = row * a.ld_out + n2;
   const rsrc_t rtw = make_rsrc(a.tw1, (uint32_t)(E::plan.twtotal + 1) * 8u);
   LP lds = lds_all + c * LROW;
 
   float2 v[P];
 #pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = src[(long)(t + T * q) * N2];
+  for (int q = 0; q < P; ++q) v[q] = fs_load<SIN>(a, ib + (long)(t + T * q) * N2);
   if constexpr (MODE == 0) E::template fft<-1>(v, lds, t, rtw);
   else E::template fft<+1>(v, lds, t, rtw);
   if constexpr (MODE == 0 || MODE == 2) {
@@ -71,7 +90,7 @@ __device__ __forceinline__ void fs_col_body(const FsArgs &a, long row, long grou
     for (int q = 0; q < P; ++q) v[q] = cscale(v[q], a.scale);
   }
 #pragma unroll
-  for (int q = 0; q < P; ++q) dst[(long)(t + T * q) * N2] = v[q];
+  for (int q = 0; q < P; ++q) fs_store<SOUT>(a, ob + (long)(t + T * q) * N2, v[q]);
 }
 
 // ---------------------------------------------------------------- row pass
@@ -80,7 +99,7 @@ __device__ __forceinline__ void fs_col_body(const FsArgs &a, long row, long grou
 // MODE 1: forward row FFT, transposed store X[k1 + N1 k2], * scale
 // MODE 2: inverse row FFT, transposed store,               * scale
 // One group of C lines k1 in [group*C, group*C + C) of one row.
-template <int L, int P, int C, int MODE, typename LP>
+template <int L, int P, int C, int MODE, bool SOUT = false, typename LP>
 __device__ __forceinline__ void fs_row_body(const FsArgs &a, long row, long group, int tid, LP lds_all)
 {
   // a line's L/P threads are consecutive; when they fit one wavefront its
@@ -117,9 +136,9 @@ __device__ __forceinline__ void fs_row_body(const FsArgs &a, long row, long grou
         v[q] = cmulc(v[q], big_twiddle(a.tlo, a.thi, a.sbits, m));
       }
     }
-    float2 *dst = a.out + row * a.ld_out + k1 * L;
+    const long ob = row * a.ld_out + k1 * L;
 #pragma unroll
-    for (int q = 0; q < P; ++q) dst[t + T * q] = v[q];
+    for (int q = 0; q < P; ++q) fs_store<SOUT>(a, ob + t + T * q, v[q]);
   } else {
     if constexpr (MODE == 1) E::template fft<-1>(v, lds, t, rtw);
     else E::template fft<+1>(v, lds, t, rtw);
@@ -134,9 +153,9 @@ __device__ __forceinline__ void fs_row_body(const FsArgs &a, long row, long grou
     const int c2 = tid % C;
     const int t2 = tid / C;
     LP lsrc = lds_all + c2 * LROW;
-    float2 *dst = a.out + row * a.ld_out + group * C + c2;
+    const long ob = row * a.ld_out + group * C + c2;
 #pragma unroll
-    for (int q = 0; q < P; ++q) dst[(long)(t2 + T * q) * N1] = lds_ld(lsrc, t2 + T * q);
+    for (int q = 0; q < P; ++q) fs_store<SOUT>(a, ob + (long)(t2 + T * q) * N1, lds_ld(lsrc, t2 + T * q));
   }
 }
 

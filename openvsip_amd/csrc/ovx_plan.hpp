@@ -57,9 +57,8 @@ struct LineFFT {
                hipStream_t s) const;
   // fastconv with X and Y in split storage (line strides in floats) and an
   // interleaved spectrum, and the plain transform on split lines, read and
-  // written by the kernels themselves; the hierarchical and register-Stockham
-  // paths (native_split())
-  bool native_split() const { return kind == HIER || kind == LDS; }
+  // written by the kernels themselves (native_split(): every transform path)
+  bool native_split() const { return kind == HIER || kind == LDS || kind == FOURSTEP; }
   int fastconv_split(const float *xr, const float *xi, long ldx, const float2 *w, long ldw, float *yr, float *yi,
                      long ldy, long lines, float scale, hipStream_t s) const;
   int transform_split(int dir, const float *in_re, const float *in_im, long ld_in, float *out_re, float *out_im,
