@@ -4,8 +4,9 @@ fftm_backend / fft_backend split overloads map to (backend.hpp:741-783; the
 reference's FFTW backend runs split DFTs, fftw.cpp:945-1001).
 
 Each split call is checked against the interleaved call of the same plan on
-the same values -- equal bit for bit, since the split path moves the floats
-exactly and runs the same kernels -- and against the FP64 oracle.  The 1-D
+the same values -- equal bit for bit, since the split kernels run the same
+arithmetic and only read and write the values as two float arrays (an exact
+move) -- and against the FP64 oracle.  The 1-D
 cases restate tests/fft.cpp:94-131 and 241-254 (sizes 64, 256, 128, 512 in
 split storage; pattern `set 2` of setup_ptr, tests/fft.cpp:57-85; < -100 dB
 against the FP64 DFT and after the inverse with scale 1/N)."""
