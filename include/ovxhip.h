@@ -176,6 +176,22 @@ int ovx_fastconv_plan_create(ovx_plan *plan, long rows, long n, float scale,
 int ovx_fastconv_exec(ovx_plan plan, const void *x, long ldx, const void *w,
                       long ldw, void *y, long ldy, long rows, void *stream);
 
+/* ---- vector-matrix multiply -------------------------------------------- */
+/* out = alpha * vmmul<dim>(v, m) over rows x cols, device pointers:
+   dim 0: out(r, c) = alpha * v(c) * m(r, c);  dim 1: out(r, c) = alpha *
+   v(r) * m(r, c).  Replaces Evaluator<op::assign<2>, be::op_expr,
+   void(LHS &, expr::Vmmul<D, V, M> const &)> (src/vsip/impl/vmmul.hpp:50-106;
+   element rule src/ovxx/expr/vmmul.hpp:53-59) in one pass.  v_complex /
+   m_complex: 1 for complex<float> data, 0 for float; out is complex<float>
+   when either is complex, else float (Promotion<T1, T2>).  Strides in
+   elements of each operand's type; any layout (row- or column-major,
+   strided subviews).  out may alias m (in place); complex products are
+   computed as (ar*br - ai*bi, ar*bi + ai*br). */
+int ovx_vmmul(int dim, int v_complex, const void *v, long v_stride,
+              int m_complex, const void *m, long m_s0, long m_s1, void *out,
+              long out_s0, long out_s1, long rows, long cols, float alpha,
+              void *stream);
+
 /* ---- replica spectra kept resident on the device ---------------------- */
 /* Transform a time-domain replica of length m (m <= n), zero-padded to n,
    once with the forward Fft (tests/fastconv.cpp:20-26 t2f) and keep the

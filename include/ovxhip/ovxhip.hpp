@@ -15,12 +15,15 @@
 #define OVXHIP_HPP
 
 #include <complex>
+#include <cstddef>
 #include <cstring>
-#include <map>
+#include <list>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <utility>
 
@@ -217,20 +220,92 @@ private:
   plan_holder p_;
 };
 
-// The Fastconv plan for (rows, n, scale, flags), made on first use and kept
-// for the calling thread (a plan is not safe for concurrent calls), so that
-// an evaluator that sees one expression shape per call -- the VSIPL++ fused
-// fastconv -- does not rebuild twiddle tables and workspaces every time.
-// The plans live until the thread ends.
-inline Fastconv &cached_fastconv(long rows, long n, float scale, unsigned flags = 0)
+// The Fastconv plan for (rows, n, scale, flags) of the calling thread, made
+// on first use, so that an evaluator that sees one expression shape per call
+// -- the VSIPL++ fused fastconv -- does not rebuild twiddle tables and
+// workspaces every time.  A plan is not safe for concurrent calls, so each
+// thread has its own.  The cache is one process-wide table under a mutex
+// (no thread-local storage: a header included after OpenVSIP's, whose
+// ovxx/thread.hpp redefines the thread_local keyword, must not depend on
+// it), keeps at most kMaxCachedPlans plans and drops the least recently used
+// one beyond that.  The caller shares ownership, so a plan dropped from the
+// cache lives until the call that holds it returns.
+class fastconv_cache {
+public:
+  static std::size_t const kMaxCachedPlans = 16;
+  static std::shared_ptr<Fastconv> get(long rows, long n, float scale, unsigned flags)
+  {
+    uint32_t bits;
+    std::memcpy(&bits, &scale, sizeof bits);
+    key_type key(std::this_thread::get_id(), rows, n, bits, flags);
+    table &t = instance();
+    {
+      std::lock_guard<std::mutex> lock(t.mutex);
+      for (auto it = t.entries.begin(); it != t.entries.end(); ++it)
+        if (it->first == key) {
+          t.entries.splice(t.entries.begin(), t.entries, it);
+          return t.entries.front().second;
+        }
+    }
+    // plan creation (device tables) runs outside the lock
+    std::shared_ptr<Fastconv> plan(new Fastconv(rows, n, scale, flags));
+    std::lock_guard<std::mutex> lock(t.mutex);
+    t.entries.emplace_front(key, plan);
+    while (t.entries.size() > kMaxCachedPlans) t.entries.pop_back();
+    return plan;
+  }
+  static std::size_t size()
+  {
+    table &t = instance();
+    std::lock_guard<std::mutex> lock(t.mutex);
+    return t.entries.size();
+  }
+  static void clear()
+  {
+    table &t = instance();
+    std::lock_guard<std::mutex> lock(t.mutex);
+    t.entries.clear();
+  }
+
+private:
+  typedef std::tuple<std::thread::id, long, long, uint32_t, unsigned> key_type;
+  struct table {
+    std::mutex mutex;
+    std::list<std::pair<key_type, std::shared_ptr<Fastconv> > > entries; // most recent first
+  };
+  static table &instance()
+  {
+    static table t;
+    return t;
+  }
+};
+
+inline std::shared_ptr<Fastconv> cached_fastconv(long rows, long n, float scale, unsigned flags = 0)
 {
-  typedef std::tuple<long, long, uint32_t, unsigned> key_type;
-  thread_local std::map<key_type, std::unique_ptr<Fastconv> > plans;
-  uint32_t bits;
-  std::memcpy(&bits, &scale, sizeof bits);
-  std::unique_ptr<Fastconv> &slot = plans[key_type(rows, n, bits, flags)];
-  if (!slot) slot.reset(new Fastconv(rows, n, scale, flags));
-  return *slot;
+  return fastconv_cache::get(rows, n, scale, flags);
+}
+
+// out = alpha * vmmul<dim>(v, m) on device data (ovx_vmmul): complex<float>
+// or float operands, strides in elements of each operand's type.
+inline void vmmul(int dim, cscalar const *v, long v_stride, cscalar const *m, long m_s0, long m_s1, cscalar *out,
+                  long out_s0, long out_s1, long rows, long cols, float alpha = 1.0f, void *stream = nullptr)
+{
+  check(ovx_vmmul(dim, 1, v, v_stride, 1, m, m_s0, m_s1, out, out_s0, out_s1, rows, cols, alpha, stream));
+}
+inline void vmmul(int dim, float const *v, long v_stride, cscalar const *m, long m_s0, long m_s1, cscalar *out,
+                  long out_s0, long out_s1, long rows, long cols, float alpha = 1.0f, void *stream = nullptr)
+{
+  check(ovx_vmmul(dim, 0, v, v_stride, 1, m, m_s0, m_s1, out, out_s0, out_s1, rows, cols, alpha, stream));
+}
+inline void vmmul(int dim, cscalar const *v, long v_stride, float const *m, long m_s0, long m_s1, cscalar *out,
+                  long out_s0, long out_s1, long rows, long cols, float alpha = 1.0f, void *stream = nullptr)
+{
+  check(ovx_vmmul(dim, 1, v, v_stride, 0, m, m_s0, m_s1, out, out_s0, out_s1, rows, cols, alpha, stream));
+}
+inline void vmmul(int dim, float const *v, long v_stride, float const *m, long m_s0, long m_s1, float *out,
+                  long out_s0, long out_s1, long rows, long cols, float alpha = 1.0f, void *stream = nullptr)
+{
+  check(ovx_vmmul(dim, 0, v, v_stride, 0, m, m_s0, m_s1, out, out_s0, out_s1, rows, cols, alpha, stream));
 }
 
 // vsip::Correlation<const_Vector, R, complex<float>> (corr.hpp:55-122).

@@ -6,6 +6,7 @@ object of the same name (paths relative to the OpenVSIP tree):
   Fftm         vsip::Fftm            src/vsip/impl/signal/fft.hpp:134-161
   Fft          vsip::Fft             src/vsip/impl/signal/fft.hpp:75-103
   Fastconv     inv_fftm(vmmul<0>(w, fwd_fftm(x)))  tests/fastconv.cpp:181-213
+  vmmul        vsip::vmmul<D>        src/vsip/impl/vmmul.hpp:50-128
   Correlation  vsip::Correlation     src/vsip/impl/signal/corr.hpp:55-122
   Convolution  vsip::Convolution     src/vsip/impl/signal/conv.hpp:55-108
   Storage      ovxx::storage_manager src/ovxx/storage/manager.hpp:207-360
@@ -212,6 +213,32 @@ class Fastconv(_Plan):
         check(lib().ovx_fastconv_exec_replica(self._h, _ptr(x), x.stride(0), replica.id, _ptr(out),
                                               out.stride(0), rows, _stream()))
         return out
+
+
+def vmmul(dim, v, m, out=None, alpha=1.0):
+    """out = alpha * vmmul<dim>(v, m): dim 0 multiplies each row of m by v
+    (len(v) == m.shape[1]), dim 1 each column (len(v) == m.shape[0]).  v and
+    m are complex64 or float32 GPU tensors of any strides; out (complex64
+    when either is complex, else float32) may be m."""
+    torch = _torch()
+    ok = (torch.complex64, torch.float32)
+    for t, name in ((v, "v"), (m, "m")):
+        if t.dtype not in ok or not t.is_cuda:
+            raise TypeError(f"{name} must be a complex64 or float32 GPU tensor")
+    if v.dim() != 1 or m.dim() != 2:
+        raise ValueError("v must be 1-D and m 2-D")
+    rows, cols = m.shape
+    if v.shape[0] != (cols if dim == 0 else rows):
+        raise ValueError(f"vmmul<{dim}>: vector length {v.shape[0]} does not match the matrix {rows}x{cols}")
+    odt = torch.complex64 if torch.complex64 in (v.dtype, m.dtype) else torch.float32
+    if out is None:
+        out = torch.empty((rows, cols), dtype=odt, device=m.device)
+    if out.dtype != odt or tuple(out.shape) != (rows, cols):
+        raise TypeError("out has the wrong type or shape")
+    check(lib().ovx_vmmul(dim, int(v.dtype == torch.complex64), _ptr(v), v.stride(0),
+                          int(m.dtype == torch.complex64), _ptr(m), m.stride(0), m.stride(1), _ptr(out),
+                          out.stride(0), out.stride(1), rows, cols, alpha, _stream()))
+    return out
 
 
 class Replica:

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "ovxhip/ovxhip.hpp"
@@ -76,11 +77,20 @@ int main()
   CHECK(same);
 
   // fused fastconv through the plan cache: y = IDFT(w . DFT(x)) / n
-  ovxhip::Fastconv &fc = ovxhip::cached_fastconv(rows, n, 1.0f / n, OVX_FLAG_HOST_POINTERS);
-  CHECK(&fc == &ovxhip::cached_fastconv(rows, n, 1.0f / n, OVX_FLAG_HOST_POINTERS));
-  CHECK(&fc != &ovxhip::cached_fastconv(rows, n, 0.5f / n, OVX_FLAG_HOST_POINTERS));
+  std::shared_ptr<ovxhip::Fastconv> fc = ovxhip::cached_fastconv(rows, n, 1.0f / n, OVX_FLAG_HOST_POINTERS);
+  CHECK(fc == ovxhip::cached_fastconv(rows, n, 1.0f / n, OVX_FLAG_HOST_POINTERS));
+  CHECK(fc != ovxhip::cached_fastconv(rows, n, 0.5f / n, OVX_FLAG_HOST_POINTERS));
+  // another thread gets its own plan (a plan is not safe for concurrent calls)
+  std::shared_ptr<ovxhip::Fastconv> other;
+  std::thread([&] { other = ovxhip::cached_fastconv(rows, n, 1.0f / n, OVX_FLAG_HOST_POINTERS); }).join();
+  CHECK(other && other != fc);
+  // the cache is bounded; a plan dropped from it stays valid for its holder
+  for (int i = 0; i < (int)ovxhip::fastconv_cache::kMaxCachedPlans + 4; ++i)
+    ovxhip::cached_fastconv(1, 64, 1.0f + i, OVX_FLAG_HOST_POINTERS);
+  CHECK(ovxhip::fastconv_cache::size() == ovxhip::fastconv_cache::kMaxCachedPlans);
+  CHECK(fc != ovxhip::cached_fastconv(rows, n, 1.0f / n, OVX_FLAG_HOST_POINTERS));
   std::vector<zf> yc(rows * n);
-  fc(x.data(), w.data(), yc.data());
+  (*fc)(x.data(), w.data(), yc.data());
   for (long r = 0; r < rows; ++r) {
     std::vector<zd> xr(x.begin() + r * n, x.begin() + (r + 1) * n);
     std::vector<zd> f = dft(xr, -1);

@@ -34,7 +34,7 @@ def test_cpp_host_api_on_gpu(tmp_path):
         pytest.fail("libovxhip.so not built")
     exe = str(tmp_path / "host_api_gpu")
     src = os.path.join(ROOT, "tests", "cpp", "host_api_gpu.cpp")
-    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src,
+    subprocess.run(["g++", "-std=c++17", "-O2", "-pthread", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src,
                     "-L", LIB_DIR, "-lovxhip", "-Wl,-rpath," + LIB_DIR, "-o", exe], check=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
