@@ -308,6 +308,10 @@ int ovx_batch_exec(ovx_batch batch, void *stream);
 int ovx_batch_exec_plain(ovx_batch batch, void *stream);
 long ovx_batch_tiles(ovx_batch batch);
 int ovx_batch_launches(ovx_batch batch);
+/* Kernel variant job `job` (in order of ovx_batch_add_fastconv) runs in the
+   committed batch -- the same numbering as OVX_FLAG_VARIANT(v + 1), so a
+   standalone plan made with that flag runs the same device body -- or -1. */
+int ovx_batch_job_variant(ovx_batch batch, long job);
 int ovx_batch_destroy(ovx_batch batch);
 
 /* ---- continuous-batching server ----------------------------------------- */

@@ -91,6 +91,7 @@ SIGNATURES = {
     "ovx_batch_exec_plain": (_I, [_P, _P]),
     "ovx_batch_tiles": (_L, [_P]),
     "ovx_batch_launches": (_I, [_P]),
+    "ovx_batch_job_variant": (_I, [_P, _L]),
     "ovx_batch_destroy": (_I, [_P]),
     "ovx_server_create": (_I, [_PP, ctypes.c_void_p]),
     "ovx_server_submit": (_I, [_P, ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulong)]),

@@ -417,6 +417,11 @@ class Batch:
     def launches(self):
         return lib().ovx_batch_launches(self._h)
 
+    def job_variants(self):
+        """Kernel variant of every job (in the order added), as Fastconv's
+        `variant` argument takes it."""
+        return [lib().ovx_batch_job_variant(self._h, k) for k in range(len(self._keep))]
+
     def close(self):
         if self._h:
             lib().ovx_batch_destroy(self._h)

@@ -386,3 +386,9 @@ extern "C" int ovx_batch_launches(ovx_batch b)
   if (!b) return -1;
   return b->gb.nlaunch;
 }
+
+extern "C" int ovx_batch_job_variant(ovx_batch b, long job)
+{
+  if (!b || !b->committed || job < 0 || job >= (long)b->paths.size()) return -1;
+  return b->paths[(size_t)job]->variant;
+}
