@@ -60,14 +60,39 @@ def parse():
 
 
 def _cpu_model():
+    """The host CPU as /proc/cpuinfo names it (full model string), with the
+    logical CPUs this process may run on."""
+    model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                    model = line.split(":", 1)[1].strip()
+                    break
     except OSError:
         pass
-    return "unknown"
+    return {"model_name": model, "logical_cpus_available": len(os.sched_getaffinity(0)),
+            "logical_cpus_total": os.cpu_count()}
+
+
+def _fftw3f_probe():
+    """Whether the CPU FFTW library the reference's fftm backend links
+    (libfftw3f, src/ovxx/fftw/fftw.cpp:73-96) is on this host.  When it is,
+    the reference's own FFT calls could be timed as the CPU baseline; when it
+    is not, the baseline is the restated FP32 port and this says so."""
+    import ctypes.util
+    import subprocess
+    found = ctypes.util.find_library("fftw3f")
+    listing = ""
+    try:
+        listing = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        pass
+    libs = sorted({ln.split("=>")[-1].strip() for ln in listing.splitlines() if "fftw" in ln})
+    return {"libfftw3f": found or "absent",
+            "fftw_libraries_listed": libs,
+            "note": "libhipfftw is ROCm's GPU FFT library with an FFTW-style API, not the CPU FFTW"
+            if any("hipfftw" in p for p in libs) else ""}
 
 
 def cpu_baseline(x, w, seconds):
@@ -103,6 +128,7 @@ def cpu_baseline(x, w, seconds):
             samples.append((time.perf_counter() - t0) / loops)
         return float(np.median(samples)), len(samples), loops
 
+    fftw = _fftw3f_probe()
     med, count, loops = median_pass(1, seconds)
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
     med_all, count_all, loops_all = median_pass(cores, max(1.0, seconds / 2))
@@ -114,6 +140,11 @@ def cpu_baseline(x, w, seconds):
         "sample": f"median of {count} samples of {loops} passes each over the full {rows}x{x.shape[1]} matrix, "
                   "four-pass FP32 CPU fastconv (FFT rows, vmmul, inverse FFT rows, 1/N pass), 1 thread",
         "cpu_model": _cpu_model(),
+        "fftw3f": fftw,
+        "why_port": "the reference's CPU fastconv runs FFTW (libfftw3f); "
+                    + ("it is not installed on this host, so the baseline is the restated FP32 four-pass port "
+                       "(oracle/ovx_oracle.c)" if fftw["libfftw3f"] == "absent" else
+                       "it is installed, but this baseline times the restated FP32 four-pass port"),
         "all_cores": {"value": rows / med_all, "unit": "rows/s", "cores": cores,
                       "sample": f"median of {count_all} samples of {loops_all} passes, rows split over "
                                 f"{cores} threads; not the reference's behaviour"},
@@ -123,8 +154,10 @@ def cpu_baseline(x, w, seconds):
 def measured_traffic(plan_name, rows, n):
     """HBM bytes per launch from the committed rocprof counter summaries
     (profiles/*_hbm_traffic.json: FETCH_SIZE and WRITE_SIZE of this kernel at
-    this shape, each calibrated against a device copy), or None when none was
-    recorded."""
+    this shape, each calibrated against a device copy) and the file they came
+    from, or (None, None) when none was recorded.  The counters cannot be read
+    inside this run (rocprofv3 --pmc is a separate profiled run of this
+    script), so the figure is the committed one."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.json"))):
         try:
@@ -133,8 +166,8 @@ def measured_traffic(plan_name, rows, n):
         except (OSError, ValueError):
             continue
         if rec.get("plan_name") == plan_name and rec.get("rows") == rows and rec.get("n") == n:
-            return rec["traffic_bytes"]
-    return None
+            return rec["traffic_bytes"], os.path.relpath(path, ROOT)
+    return None, None
 
 
 def drop_in_section(args, x, w):
@@ -547,6 +580,7 @@ def main():
     rel = oracle.rel_l2(y[idx], ref)
 
     sec = ms / 1e3
+    traffic, traffic_src = measured_traffic(fc.name, R, N)
     alg_bytes = 2 * R * N * 8 + N * 8
     flops = 2 * R * 5 * N * np.log2(N) + 6 * R * N
     achieved = alg_bytes / sec / 1e9
@@ -568,7 +602,8 @@ def main():
                    "rows": R, "n": N, "kernel": fc.name,
                    "autotuned_variant": None if tuned is None else tuned[0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(fc.name, R, N)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src or "no committed counter profile for this kernel and shape"},
         "hbm_copy": {"what": "device copy of X into Y (2*R*N*8 bytes), same timing: the practical "
                              "ceiling of a pass that reads X once and writes Y once",
                      "ms": copy_ms, "GBs": 2 * R * N * 8 / (copy_ms * 1e-3) / 1e9,

@@ -108,6 +108,17 @@ int ovx_length_variants(long n);
 int ovx_autotune_fastconv(long n, long rows, int *best_variant, double *best_us);
 /* Cached variant for (n, rows), or -1. */
 int ovx_tuned_variant(long n, long rows);
+/* Batch tiles (no reference counterpart): for a job of rows x n inside a
+   batch or the server, time every tile body the persistent kernel holds for
+   length n (the hierarchical body and the register-Stockham body with the
+   batch's values per thread) as a one-job batch replayed from its graph, and
+   record the fastest under (n, rows rounded up to a power of two) in the
+   same cache.  ovx_batch_commit tunes the keys it lacks (unless the
+   environment sets OVX_NO_BATCH_TUNE); the server uses the cached choices.
+   A length with one tile body returns OVX_ERR_UNSUPPORTED. */
+int ovx_autotune_batch(long n, long rows, int *best_variant, double *best_us);
+/* Cached batch tile variant for (n, rows), or -1. */
+int ovx_tuned_batch_variant(long n, long rows);
 
 /* ---- Fftm (multiple 1-D transforms) ----------------------------------- */
 /* Create a plan for a rows x cols Fftm along `axis` in `direction`, result

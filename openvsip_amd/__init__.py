@@ -8,4 +8,4 @@ from ._lib import OvxError, OvxInvalid, OvxLibraryMissing, OvxUnsupported, LIB_P
 from .api import (COL, FFT_FWD, FFT_INV, ROW, Batch, Convolution, Correlation, Fastconv, Fft,  # noqa: F401
                   Fftm, Replica, init, variants)
 from .api import CorrReference, Server, corr_request, fastconv_request, fftm_request  # noqa: F401
-from .api import FftmReal, Storage, autotune, tuned_variant, vmmul  # noqa: F401
+from .api import FftmReal, Storage, autotune, autotune_batch, tuned_batch_variant, tuned_variant, vmmul  # noqa: F401

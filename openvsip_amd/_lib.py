@@ -55,6 +55,8 @@ SIGNATURES = {
     "ovx_length_variants": (_I, [_L]),
     "ovx_autotune_fastconv": (_I, [_L, _L, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
     "ovx_tuned_variant": (_I, [_L, _L]),
+    "ovx_autotune_batch": (_I, [_L, _L, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
+    "ovx_tuned_batch_variant": (_I, [_L, _L]),
     "ovx_fftm_plan_create": (_I, [_PP, _L, _L, _I, _I, _F, _U]),
     "ovx_fftm_out_of_place": (_I, [_P, _P, _L, _L, _P, _L, _L, _L, _L, _P]),
     "ovx_fftm_in_place": (_I, [_P, _P, _L, _L, _L, _L, _P]),

@@ -91,6 +91,19 @@ def tuned_variant(n, rows):
     return lib().ovx_tuned_variant(n, rows)
 
 
+def autotune_batch(n, rows):
+    """Time the batch tile bodies of length n on a rows x n job; cache and
+    return (best variant, microseconds per one-job batch replay)."""
+    v = ctypes.c_int()
+    us = ctypes.c_double()
+    check(lib().ovx_autotune_batch(n, rows, ctypes.byref(v), ctypes.byref(us)))
+    return v.value, us.value
+
+
+def tuned_batch_variant(n, rows):
+    return lib().ovx_tuned_batch_variant(n, rows)
+
+
 def _flags(variant):
     return 0 if variant is None else _lib.FLAG_VARIANT(variant + 1)
 

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -19,20 +20,27 @@
 namespace ovx {
 
 // ---------------------------------------------------------------- shared paths
-// One default kernel path (twiddle tables) per transform length, shared by
+// Kernel paths (twiddle tables) per (transform length, variant), shared by
 // batches, replicas and the server.
 static std::mutex g_path_mu;
-static std::map<long, std::unique_ptr<LineFFT>> g_paths;
+static std::map<std::pair<long, int>, std::unique_ptr<LineFFT>> g_paths;
 
-// Variant the batch tiles of length n run: the 3 * 2^k lengths use the
-// 12-values-per-thread register-Stockham configuration (variant 1), which
-// fits the persistent kernel's 1024-thread register budget.
-int batch_variant(long n) { return (n == 3072 || n == 6144 || n == 12288) ? 1 : LineFFT::default_variant(n); }
+// Variant the batch tiles of an (n, rows) job run: the autotuner's batch
+// entry for (n, rows) when it has one, else the default -- for the 3 * 2^k
+// lengths the 12-values-per-thread register-Stockham configuration (variant
+// 1), which fits the persistent kernel's 1024-thread register budget, for
+// the others the standalone default.
+int batch_variant(long n, long rows)
+{
+  int v = tuned_batch_variant(n, rows);
+  if (v >= 0) return v;
+  return (n == 3072 || n == 6144 || n == 12288) ? 1 : LineFFT::default_variant(n);
+}
 
-const LineFFT *shared_path(long n, int *status)
+const LineFFT *shared_path(long n, int variant, int *status)
 {
   std::lock_guard<std::mutex> g(g_path_mu);
-  auto it = g_paths.find(n);
+  auto it = g_paths.find({ n, variant });
   if (it != g_paths.end()) {
     *status = OVX_OK;
     return it->second.get();
@@ -42,12 +50,14 @@ const LineFFT *shared_path(long n, int *status)
     *status = fail(OVX_ERR_NOMEM, "host allocation");
     return nullptr;
   }
-  *status = f->init(n, batch_variant(n));
+  *status = f->init(n, variant);
   if (*status) return nullptr;
   const LineFFT *p = f.get();
-  g_paths[n] = std::move(f);
+  g_paths[{ n, variant }] = std::move(f);
   return p;
 }
+
+const LineFFT *shared_path(long n, long rows, int *status) { return shared_path(n, batch_variant(n, rows), status); }
 
 int batch_path_of(const LineFFT &f)
 {
@@ -321,10 +331,20 @@ extern "C" int ovx_batch_commit(ovx_batch b, void *stream)
   b->djobs.clear();
   b->paths.clear();
   b->preps.clear();
+  // tile bodies for the (n, rows) keys the autotuner has not seen yet
+  if (!getenv("OVX_NO_BATCH_TUNE")) {
+    for (size_t i = 0; i < b->jobs.size(); ++i) {
+      const auto &h = b->jobs[i];
+      if (batch_candidates(h.n).size() > 1 && tuned_batch_variant(h.n, h.rows) < 0) {
+        int st = autotune_batch(h.n, h.rows, nullptr, nullptr);
+        if (st) return st;
+      }
+    }
+  }
   for (size_t i = 0; i < b->jobs.size(); ++i) {
     const auto &h = b->jobs[i];
     int st;
-    const LineFFT *f = shared_path(h.n, &st);
+    const LineFFT *f = shared_path(h.n, h.rows, &st);
     if (!f) return st;
     const float2 *prep = nullptr;
     long rn = 0;

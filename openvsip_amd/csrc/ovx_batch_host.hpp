@@ -9,8 +9,19 @@
 
 namespace ovx {
 
-// Default kernel path (and its twiddles) of length n, created on first use.
-const LineFFT *shared_path(long n, int *status);
+// Kernel path (and its twiddles) of length n in `variant`, created on first
+// use and shared.  The (n, rows) form takes batch_variant(n, rows).
+const LineFFT *shared_path(long n, int variant, int *status);
+const LineFFT *shared_path(long n, long rows, int *status);
+// Variant of the tile bodies an (n, rows) job runs in a batch or the server:
+// the autotuner's batch entry, else the default.
+int batch_variant(long n, long rows);
+// Batch tile autotuner (ovx_tune.cpp): the variants whose device body a
+// batch tile can run for length n, the cached choice for (n, rows) or -1,
+// and the tuning itself (records the choice).
+std::vector<int> batch_candidates(long n);
+int tuned_batch_variant(long n, long rows);
+int autotune_batch(long n, long rows, int *best_variant, double *best_us);
 int batch_path_of(const LineFFT &f);
 int device_cu_count();
 int make_batch_job(const LineFFT &f, long rows, const float2 *x, long ldx, float2 *y, long ldy, const float2 *wprep,

@@ -21,6 +21,7 @@
 #include "ovx_fourstep.hpp"
 #include "ovx_hier.hpp"
 #include "ovx_internal.hpp"
+#include "ovx_batch_host.hpp"
 #include "ovx_plan.hpp"
 
 using namespace ovx;
@@ -271,7 +272,7 @@ int LineFFT::transform_ws(int dir, const float2 *in, long ld_in, float2 *out, lo
 // `a`): a single spectrum is permuted once into the kernel's [k1][k2] order
 // with the scale folded in; a coefficient matrix is read in natural order.
 int LineFFT::hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *w, long ldw, long ldy, long lines,
-                           float scale, hipStream_t s) const
+                           float scale, hipStream_t s, const ExtScratch *ext) const
 {
   if (!offsets_fit(hier_rows_per_block(n), std::max(std::max(ldx, ldy), ldw)))
     return fail(OVX_ERR_UNSUPPORTED, "row stride too large for 32-bit buffer offsets");
@@ -279,11 +280,18 @@ int LineFFT::hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *
   a.ldy = ldy;
   a.rows = lines;
   if (ldw == 0) {
-    int st = wp.alloc((size_t)n * 8);
-    if (st) return st;
-    hipError_t e = launch_hier_permute(w, 0, (float2 *)wp.p, n, 1, scale, s);
+    float2 *wpp;
+    if (ext) {
+      if (ext->wp_elems < (size_t)n) return fail(OVX_ERR_UNSUPPORTED, "lent scratch too small for the spectrum");
+      wpp = ext->wp;
+    } else {
+      int st = wp.alloc((size_t)n * 8);
+      if (st) return st;
+      wpp = (float2 *)wp.p;
+    }
+    hipError_t e = launch_hier_permute(w, 0, wpp, n, 1, scale, s);
     if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
-    a.w = (const float2 *)wp.p;
+    a.w = wpp;
     a.ldw = 0;
     a.natural = 0;
     a.scale = 1.0f;
@@ -295,6 +303,13 @@ int LineFFT::hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *
   }
   hipError_t e = launch_hier(launch_kind, n, a, s);
   return e == hipSuccess ? OVX_OK : hip_fail(e, "fastconv launch");
+}
+
+size_t LineFFT::fastconv_scratch_wp(long lines, long ldw) const
+{
+  if (kind == HIER && ldw == 0) return (size_t)n;
+  if (kind == FOURSTEP) return (size_t)(ldw ? lines : 1) * (size_t)n;
+  return 0;
 }
 
 int LineFFT::transform_split(int dir, const float *in_re, const float *in_im, long ld_in, float *out_re,
@@ -449,7 +464,7 @@ int LineFFT::fastconv_split(const float *xr, const float *xi, long ldx, const fl
 }
 
 int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, float2 *y, long ldy, long lines,
-                      float scale, hipStream_t s) const
+                      float scale, hipStream_t s, const ExtScratch *ext) const
 {
   if (lines <= 0) return OVX_OK;
   if (!tables_ready()) return fail(OVX_ERR_STATE, "fastconv plan has no twiddle tables");
@@ -457,7 +472,7 @@ int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, floa
     HierArgs a = hier_args(*this);
     a.x = x;
     a.y = y;
-    return hier_fastconv(a, HIER_FASTCONV, ldx, w, ldw, ldy, lines, scale, s);
+    return hier_fastconv(a, HIER_FASTCONV, ldx, w, ldw, ldy, lines, scale, s, ext);
   }
   if (kind == LDS) {
     long ld = std::max(std::max(ldx, ldy), ldw);
@@ -470,11 +485,18 @@ int LineFFT::fastconv(const float2 *x, long ldx, const float2 *w, long ldw, floa
   if (kind == FOURSTEP) {
     if (lines > 65535) return fail(OVX_ERR_UNSUPPORTED, "more than 65535 lines of a four-step fastconv");
     long wrows = ldw ? lines : 1;
-    int st = wp.alloc((size_t)wrows * n * 8);
-    if (st) return st;
-    hipError_t e = launch_fs_permute(w, ldw, (float2 *)wp.p, n1, n2, wrows, scale, s);
+    float2 *wpp;
+    if (ext) {
+      if (ext->wp_elems < (size_t)wrows * n) return fail(OVX_ERR_UNSUPPORTED, "lent scratch too small for the spectrum");
+      wpp = ext->wp;
+    } else {
+      int st = wp.alloc((size_t)wrows * n * 8);
+      if (st) return st;
+      wpp = (float2 *)wp.p;
+    }
+    hipError_t e = launch_fs_permute(w, ldw, wpp, n1, n2, wrows, scale, s);
     if (e != hipSuccess) return hip_fail(e, "spectrum permutation");
-    return fs_fastconv(x, ldx, (const float2 *)wp.p, ldw ? n : 0, y, ldy, lines, s);
+    return fs_fastconv(x, ldx, wpp, ldw ? n : 0, y, ldy, lines, s);
   }
   return fail(OVX_ERR_UNSUPPORTED, "no fastconv path");
 }
@@ -900,18 +922,54 @@ extern "C" int ovx_fastconv_plan_create(ovx_plan *plan, long rows, long n, float
   return OVX_OK;
 }
 
+// A plan's device scratch is shared by its calls, which are ordered on one
+// stream: a call on another stream while the last call may still be running
+// is refused (OVX_ERR_STATE) instead of racing on that scratch.
+static int claim_plan(ovx_plan p, hipStream_t s)
+{
+  if (!p->busy || p->busy_stream == s) return OVX_OK;
+  hipError_t q = hipEventQuery(p->busy);
+  if (q == hipErrorNotReady)
+    return fail(OVX_ERR_STATE, "plan still in use by a call on another stream (a plan is not safe for concurrent calls)");
+  if (q != hipSuccess) return hip_fail(q, "event query");
+  return OVX_OK;
+}
+
+static int mark_plan(ovx_plan p, hipStream_t s, int status)
+{
+  if (status) return status;
+  hipError_t e = hipSuccess;
+  if (!p->busy) e = hipEventCreateWithFlags(&p->busy, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(p->busy, s);
+  if (e != hipSuccess) return hip_fail(e, "event record");
+  p->busy_stream = s;
+  return OVX_OK;
+}
+
+static int fastconv_exec_body(ovx_plan p, const void *x, long ldx, const void *w, long ldw, void *y, long ldy,
+                                long rows, hipStream_t s);
+
 extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void *w, long ldw, void *y, long ldy,
                                  long rows, void *stream)
 {
   int st = check_plan(p, PLAN_FASTCONV);
   if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if (p->flags & OVX_FLAG_HOST_POINTERS) return fastconv_exec_body(p, x, ldx, w, ldw, y, ldy, rows, s);
+  if ((st = claim_plan(p, s))) return st;
+  return mark_plan(p, s, fastconv_exec_body(p, x, ldx, w, ldw, y, ldy, rows, s));
+}
+
+static int fastconv_exec_body(ovx_plan p, const void *x, long ldx, const void *w, long ldw, void *y, long ldy,
+                                long rows, hipStream_t s)
+{
+  int st;
   if (rows < 0 || rows > p->rows) return fail(OVX_ERR_INVALID, "rows %ld outside the plan's %ld", rows, p->rows);
   long n = p->cols;
   if (rows > 1 && (ldx < n || ldy < n)) return fail(OVX_ERR_INVALID, "row stride shorter than n");
   if (ldw != 0 && rows > 1 && ldw < n) return fail(OVX_ERR_INVALID, "coefficient row stride shorter than n");
   if (rows == 0) return OVX_OK; // empty views may carry null pointers
   if (!x || !w || !y) return fail(OVX_ERR_INVALID, "null data pointer");
-  hipStream_t s = (hipStream_t)stream;
   if (p->bf16x3) {
     if (ldw != 0) return fail(OVX_ERR_UNSUPPORTED, "BF16x3 variant takes one spectrum vector");
     if (!p->line.tables_ready() || !p->u256.p) return fail(OVX_ERR_STATE, "BF16x3 tables missing");
@@ -1073,9 +1131,12 @@ extern "C" int ovx_fastconv_split_exec(ovx_plan p, const void *x_re, const void 
     e = launch_deinterleave(wx, n, y.re, y.im, ldy, rows, n, s);
     return e == hipSuccess ? OVX_OK : hip_fail(e, "deinterleave");
   };
-  if (!(p->flags & OVX_FLAG_HOST_POINTERS))
-    return run({(const float *)x_re, (const float *)x_im}, {(const float *)w_re, (const float *)w_im},
-               {(float *)y_re, (float *)y_im});
+  if (!(p->flags & OVX_FLAG_HOST_POINTERS)) {
+    if ((st = claim_plan(p, s))) return st;
+    return mark_plan(p, s,
+                     run({(const float *)x_re, (const float *)x_im}, {(const float *)w_re, (const float *)w_im},
+                         {(float *)y_re, (float *)y_im}));
+  }
   const HostLines gx = host_lines(rows, ldx, n, 4), gw = host_lines(wlines, ldw, n, 4),
                   gy = host_lines(rows, ldy, n, 4);
   if ((st = p->stage_in.alloc(2 * gx.extent())) || (st = p->stage_w.alloc(2 * gw.extent())) ||
@@ -1176,7 +1237,7 @@ namespace {
 struct Replica {
   long n = 0;
   DevBuf spec, prep, nat; // nat: natural order, 1/n folded in (register-Stockham paths)
-  int prep_kind = 0, prep_variant = -1;
+  int prep_kind = 0;
 };
 std::mutex g_rep_mu;
 std::map<int, std::unique_ptr<Replica>> g_replicas;
@@ -1213,7 +1274,6 @@ extern "C" int ovx_replica_create(long n, const void *replica, long m, int is_sp
   hipError_t e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) return hip_fail(e, "replica preparation");
   r->prep_kind = f.kind;
-  r->prep_variant = f.variant;
   if (f.kind != LineFFT::LDS && f.kind != LineFFT::FOURSTEP) {
     // a tuned plan may run this length on the register-Stockham path, which
     // reads the spectrum in natural order
@@ -1255,14 +1315,15 @@ int replica_prepared(int id, const float2 **prep, long *n, const LineFFT *f)
   auto it = g_replicas.find(id);
   if (it == g_replicas.end()) return fail(OVX_ERR_INVALID, "unknown replica id %d", id);
   // register-Stockham paths read the spectrum in natural order whatever
-  // their variant; the other paths need the layout of their own variant
+  // their variant; the hierarchical kernels read the [k1][k2] order, the
+  // four-step kernels theirs: one layout per kind
   const Replica &r = *it->second;
   if (f->kind == LineFFT::LDS && r.prep_kind != LineFFT::LDS && r.nat.p) {
     *prep = (const float2 *)r.nat.p;
     *n = r.n;
     return OVX_OK;
   }
-  bool ok = r.prep_kind == f->kind && (f->kind == LineFFT::LDS || r.prep_variant == f->variant);
+  bool ok = r.prep_kind == f->kind;
   if (!ok) return fail(OVX_ERR_STATE, "replica %d is not prepared for this kernel path", id);
   *prep = (const float2 *)r.prep.p;
   *n = it->second->n;
@@ -1293,9 +1354,10 @@ extern "C" int ovx_fastconv_exec_replica(ovx_plan p, const void *x, long ldx, in
   hipStream_t s = (hipStream_t)stream;
   if (rows == 0) return OVX_OK;
   if (p->bf16x3) return ovx_fastconv_exec(p, x, ldx, spec, 0, y, ldy, rows, stream);
+  if ((st = claim_plan(p, s))) return st;
   if (p->scale == 1.0f / (float)n && replica_prepared(replica, &prep, &rn, &p->line) == OVX_OK)
-    return p->line.fastconv_prepared((const float2 *)x, ldx, prep, (float2 *)y, ldy, rows, s);
-  return p->line.fastconv((const float2 *)x, ldx, spec, 0, (float2 *)y, ldy, rows, p->scale, s);
+    return mark_plan(p, s, p->line.fastconv_prepared((const float2 *)x, ldx, prep, (float2 *)y, ldy, rows, s));
+  return mark_plan(p, s, p->line.fastconv((const float2 *)x, ldx, spec, 0, (float2 *)y, ldy, rows, p->scale, s));
 }
 
 // ---------------------------------------------------------------- plans

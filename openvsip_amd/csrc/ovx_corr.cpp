@@ -94,7 +94,11 @@ static int ols_run(CorrState &c, const float2 *in, float2 *out, int conj_input, 
 
 using namespace ovx;
 
-ovx_plan_s::~ovx_plan_s() { delete corr; }
+ovx_plan_s::~ovx_plan_s()
+{
+  delete corr;
+  if (busy) (void)hipEventDestroy(busy);
+}
 
 const char *ovx_plan_s::name() const
 {

@@ -26,6 +26,14 @@ struct DevBuf {
   void release();
 };
 
+// Device scratch a caller lends to a fastconv call in place of the path's
+// own buffer (allocated on first use) for the permuted spectrum.  A call
+// that needs more than it lends fails with OVX_ERR_UNSUPPORTED.
+struct ExtScratch {
+  float2 *wp = nullptr;
+  size_t wp_elems = 0;
+};
+
 // A line transform of length n: picks the kernel path (LDS-resident rows,
 // or the two-pass four-step for longer lines) and owns its twiddles.
 struct LineFFT {
@@ -54,7 +62,10 @@ struct LineFFT {
   int transform_ws(int dir, const float2 *in, long ld_in, float2 *out, long ld_out, long lines, float scale,
                    float2 *ws, hipStream_t s) const;
   int fastconv(const float2 *x, long ldx, const float2 *w, long ldw, float2 *y, long ldy, long lines, float scale,
-               hipStream_t s) const;
+               hipStream_t s, const ExtScratch *ext = nullptr) const;
+  // permuted-spectrum elements fastconv() needs for `lines` lines and
+  // spectrum row stride ldw (0 when it permutes nothing)
+  size_t fastconv_scratch_wp(long lines, long ldw) const;
   // fastconv with X and Y in split storage (line strides in floats) and an
   // interleaved spectrum, and the plain transform on split lines, read and
   // written by the kernels themselves (native_split(): every transform path)
@@ -66,7 +77,7 @@ struct LineFFT {
   int fastconv_split_vec(const float *xr, const float *xi, long ldx, const float *wr, const float *wi, float *yr,
                          float *yi, long ldy, long lines, float scale, hipStream_t s) const;
   int hier_fastconv(HierArgs a, int launch_kind, long ldx, const float2 *w, long ldw, long ldy, long lines,
-                    float scale, hipStream_t s) const;
+                    float scale, hipStream_t s, const ExtScratch *ext = nullptr) const;
   int fs_fastconv(const float2 *x, long ldx, const float2 *wprep, long ldwp, float2 *y, long ldy, long lines,
                   hipStream_t s) const;
   // Spectrum layout the fastconv kernels of this path read: wp = scale * w
@@ -101,6 +112,11 @@ struct ovx_plan_s {
   ovx::DevBuf u256;    // its 256th roots of unity
   ovx::DevBuf wtab;    // real Fftm: exp(-2 pi i k / n), k <= n/4
   ovx::DevBuf split_ws; // split-complex calls: interleaved copy of the lines
+  // the stream of the plan's last device-pointer fastconv and an event
+  // recorded after it (its scratch -- the permuted spectrum -- is shared by
+  // the plan's calls)
+  hipEvent_t busy = nullptr;
+  hipStream_t busy_stream = nullptr;
 
   ~ovx_plan_s();
   const char *name() const;

@@ -266,7 +266,7 @@ int Server::admit_and_launch(Slot &s, std::vector<Req *> &batch)
         q[0].push_back(BatchTile{ (int)i, BT_OLS, (int)sgm, 0 });
       continue;
     }
-    const LineFFT *f = shared_path(r.n, &st);
+    const LineFFT *f = shared_path(r.n, r.rows, &st);
     if (!f) return st;
     if (r.kind == OVX_REQ_FASTCONV) {
       const float2 *prep;
@@ -323,7 +323,7 @@ int Server::launch_plain(Slot &s, Req &q)
     if (e != hipSuccess) return hip_fail(e, "correlation launch");
     return OVX_OK;
   }
-  const LineFFT *f = shared_path(r.n, &st);
+  const LineFFT *f = shared_path(r.n, r.rows, &st);
   if (!f) return st;
   if (r.kind == OVX_REQ_FASTCONV) {
     const float2 *prep;
@@ -364,7 +364,7 @@ void Server::scheduler()
         const CorrRef *cr = nullptr;
         int st = OVX_OK;
         if (r->r.kind == OVX_REQ_CORR) st = corr_ref_get(r->r.ref, &cr);
-        else f = shared_path(r->r.n, &st);
+        else f = shared_path(r->r.n, r->r.rows, &st);
         if (st) {
           queue.pop_front();
           r->status = st;
@@ -470,7 +470,7 @@ static int prepare_request(const ovx_request &r, Req &q)
   if (r.rows > 1 && (r.ldx < r.n || r.ldy < r.n)) return fail(OVX_ERR_INVALID, "row stride shorter than n");
   q.footprint = (size_t)r.rows * (size_t)(r.ldx + r.ldy) * 8;
   int st;
-  const LineFFT *f = shared_path(r.n, &st);
+  const LineFFT *f = shared_path(r.n, r.rows, &st);
   if (!f) return st;
   if (r.kind == OVX_REQ_FFTM && f->kind == LineFFT::FOURSTEP) q.ws_need = (size_t)r.rows * r.n * 8;
   q.footprint += q.ws_need;

@@ -10,10 +10,15 @@
 // clocks take about that long to settle under this load, then five
 // interleaved rounds, best round per variant -- and records the winner under
 // (n, rows rounded up to a power of two).
-// Plans created with the default variant flag consult the cache.  The cache
-// is a text file, one "n rows_bucket variant microseconds" line per entry,
-// at $OVX_TUNE_CACHE (default ~/.cache/ovxhip/tune.txt); it is the only state
-// the library persists.
+// Plans created with the default variant flag consult the cache.  Batches
+// and the server have their own entries: which tile body (hierarchical, or
+// the register-Stockham configuration the persistent kernel holds) runs the
+// rows of an (n, rows) job inside a batch, timed as a one-job batch replayed
+// from its graph; a batch tunes the keys it lacks when it is committed.  The
+// cache is a text file, one "n rows_bucket variant microseconds" line per
+// plan entry and one "b n rows_bucket variant microseconds" line per batch
+// entry, at $OVX_TUNE_CACHE (default ~/.cache/ovxhip/tune.txt); it is the
+// only state the library persists.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,12 +34,17 @@
 #include <utility>
 #include <vector>
 
+#include "ovx_batch.hpp"
+#include "ovx_batch_host.hpp"
 #include "ovx_plan.hpp"
 
 namespace ovx {
 
 static std::mutex g_tune_mu;
+// (n, rows bucket) -> (variant, microseconds): standalone fastconv plans
 static std::map<std::pair<long, long>, std::pair<int, double>> g_tune;
+// the same for batch and server tiles ("b" lines in the cache file)
+static std::map<std::pair<long, long>, std::pair<int, double>> g_tune_batch;
 static bool g_tune_loaded = false;
 
 static long rows_bucket(long rows)
@@ -58,10 +68,14 @@ static void load_locked()
   g_tune_loaded = true;
   FILE *f = fopen(cache_path().c_str(), "r");
   if (!f) return;
-  long n, rb;
-  int v;
-  double us;
-  while (fscanf(f, "%ld %ld %d %lf", &n, &rb, &v, &us) == 4) g_tune[{ n, rb }] = { v, us };
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    long n, rb;
+    int v;
+    double us;
+    if (sscanf(line, "b %ld %ld %d %lf", &n, &rb, &v, &us) == 4) g_tune_batch[{ n, rb }] = { v, us };
+    else if (sscanf(line, "%ld %ld %d %lf", &n, &rb, &v, &us) == 4) g_tune[{ n, rb }] = { v, us };
+  }
   fclose(f);
 }
 
@@ -76,6 +90,8 @@ static void save_locked()
   FILE *f = fopen(tmp.c_str(), "w");
   if (!f) return;
   for (auto &e : g_tune) fprintf(f, "%ld %ld %d %.3f\n", e.first.first, e.first.second, e.second.first, e.second.second);
+  for (auto &e : g_tune_batch)
+    fprintf(f, "b %ld %ld %d %.3f\n", e.first.first, e.first.second, e.second.first, e.second.second);
   fclose(f);
   rename(tmp.c_str(), path.c_str());
 }
@@ -208,3 +224,120 @@ extern "C" int ovx_autotune_fastconv(long n, long rows, int *best_variant, doubl
 }
 
 extern "C" int ovx_tuned_variant(long n, long rows) { return tuned_variant(n, rows); }
+
+// ---------------------------------------------------------------- batch tiles
+// The tile bodies a batch (and the server) can run for length n: the
+// hierarchical body and the register-Stockham body with batch_row_p(n)
+// values per thread, each the device body of one standalone variant.
+namespace ovx {
+
+std::vector<int> batch_candidates(long n)
+{
+  std::vector<int> out;
+  if (n < 1 || n > kMaxRowN) return out;
+  const bool hier = hier_supported(n);
+  if (hier && batch_supported(BP_HIER, (int)n)) out.push_back(0);
+  if (batch_supported(BP_ROW, (int)n)) {
+    for (int v = 0; v < row_variants((int)n); ++v) {
+      RowConfig rc;
+      if (row_config((int)n, v, &rc) && rc.p == batch_row_p((int)n)) out.push_back(hier ? v + 1 : v);
+    }
+  }
+  return out;
+}
+
+int tuned_batch_variant(long n, long rows)
+{
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  load_locked();
+  auto it = g_tune_batch.find({ n, rows_bucket(rows) });
+  return it == g_tune_batch.end() ? -1 : it->second.first;
+}
+
+// Time one batch of a single job (n x rows, zero data, a spectrum prepared
+// for the candidate's path) per candidate tile body, replayed from its
+// captured graph after ~20 ms of untimed replays; the fastest of five
+// rounds of 20 replays counts.
+int autotune_batch(long n, long rows, int *best_variant, double *best_us)
+{
+  std::vector<int> cands = batch_candidates(n);
+  if (cands.empty()) return fail(OVX_ERR_UNSUPPORTED, "length %ld has no batch tile to tune", n);
+  DevBuf x, y, w, wp;
+  const size_t bytes = (size_t)rows * n * 8;
+  int st;
+  if ((st = x.alloc(bytes)) || (st = y.alloc(bytes)) || (st = w.alloc((size_t)n * 8)) || (st = wp.alloc((size_t)n * 8)))
+    return st;
+  hipError_t e = hipMemset(x.p, 0, bytes);
+  if (e == hipSuccess) e = hipMemset(w.p, 0, (size_t)n * 8);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "batch tuning buffers");
+  hipStream_t s;
+  if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "tuning stream");
+  int best = -1;
+  double best_t = 1e30;
+  for (int v : cands) {
+    LineFFT f;
+    if ((st = f.init(n, v))) break;
+    if ((st = f.prepare_spectrum((const float2 *)w.p, (float2 *)wp.p, 1.0f / (float)n, s))) break;
+    BatchJob j;
+    if ((st = make_batch_job(f, rows, (const float2 *)x.p, n, (float2 *)y.p, n, (const float2 *)wp.p, &j))) break;
+    std::vector<BatchJob> jobs(1, j);
+    std::vector<BatchTile> q[kBatchQueues];
+    make_batch_tiles(j, 0, q);
+    std::vector<TileQueue> queues;
+    queues.push_back(TileQueue{ 0, 0, std::move(q[0]) });
+    GraphBatch gb;
+    if ((st = gb.build(jobs, queues, s))) break;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < 400 && !st; ++k) {
+      st = gb.launch(s);
+      if ((k & 7) == 7) {
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) st = hip_fail(e, "batch tuning");
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) break;
+      }
+    }
+    double t_min = 1e30;
+    for (int round = 0; round < 5 && !st; ++round) {
+      hipEvent_t e0, e1;
+      if (hipEventCreate(&e0) != hipSuccess) { st = fail(OVX_ERR_DEVICE, "event create"); break; }
+      if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); st = fail(OVX_ERR_DEVICE, "event create"); break; }
+      e = hipEventRecord(e0, s);
+      for (int k = 0; k < 20 && !st; ++k) st = gb.launch(s);
+      if (e == hipSuccess) e = hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (!st && e != hipSuccess) st = hip_fail(e, "batch tuning timing");
+      if (!st) t_min = std::min(t_min, ms * 1e3 / 20);
+    }
+    if (st) break;
+    if (t_min < best_t) {
+      best_t = t_min;
+      best = v;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipStreamDestroy(s);
+  if (st) return st;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    load_locked();
+    g_tune_batch[{ n, rows_bucket(rows) }] = { best, best_t };
+    save_locked();
+  }
+  if (best_variant) *best_variant = best;
+  if (best_us) *best_us = best_t;
+  return OVX_OK;
+}
+
+} // namespace ovx
+
+extern "C" int ovx_autotune_batch(long n, long rows, int *best_variant, double *best_us)
+{
+  if (n < 1 || rows < 1) return fail(OVX_ERR_INVALID, "bad shape");
+  return autotune_batch(n, rows, best_variant, best_us);
+}
+
+extern "C" int ovx_tuned_batch_variant(long n, long rows) { return tuned_batch_variant(n, rows); }

@@ -50,15 +50,14 @@ def test_batch_matches_oracle_and_standalone(ovx):
     assert b.launches == 4
     b()
     outs = [host(yd) for *_, yd in jobs]
-    for (n, rows, x, xd, yd), y in zip(jobs, outs):
-        print("job", n, rows, flush=True)
+    variants = b.job_variants()
+    for (n, rows, x, xd, yd), y, variant in zip(jobs, outs, variants):
+        print("job", n, rows, "variant", variant, flush=True)
         assert np.isfinite(y).all(), (n, rows)
         err = o.rel_l2(y, o.fastconv64(x, reps[n][1]))
         assert err < 1e-5, (n, rows, err)
-        # the batch runs the 3 * 2^k lengths with 12 values per thread
-        # (variant 1 of those lengths) and 2048 / 4096 on the register-Stockham
-        # kernel (variant 1); every other length on variant 0
-        variant = 1 if n in (3072, 6144, 12288, 2048, 4096) else 0
+        # the tile body of every job is the device body of one standalone
+        # variant (the autotuner's batch choice for its (n, rows))
         alone = ovx.Fastconv(rows, n, variant=variant).with_replica(xd, reps[n][0])
         assert np.array_equal(host(alone), y), (n, rows)
     # replay: same results again (graph replays are independent)
