@@ -187,6 +187,28 @@ int ovx_fastconv_plan_create(ovx_plan *plan, long rows, long n, float scale,
 int ovx_fastconv_exec(ovx_plan plan, const void *x, long ldx, const void *w,
                       long ldw, void *y, long ldy, long rows, void *stream);
 
+/* ---- fused fast convolution without allocation -------------------------- */
+/* No reference counterpart.  For callers that cannot allocate device memory
+   -- code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc: ovx_fastconv_exec on device pointers (same arguments, same
+   results bit for bit) with the call's scratch -- the spectrum that a
+   hierarchical or four-step plan permutes into its kernel's order -- in a
+   fixed device array of OVX_NOALLOC_SCRATCH_BYTES that the library holds on
+   each GPU until the process ends, instead of a plan buffer made on first
+   use.  The call allocates nothing (no hipMalloc, no hipFree; the plan's
+   tables were made by ovx_fastconv_plan_create).  A call that would need
+   more scratch (a four-step plan with a coefficient matrix of many rows, or
+   a spectrum longer than 2^19 points) fails with OVX_ERR_UNSUPPORTED;
+   host-pointer plans are refused.  Every call on one GPU shares that GPU's
+   array and leaves its last contents there (each call writes all of the
+   scratch it reads before reading it): calls that overlap on one GPU --
+   other streams, other threads, other plans -- race, and must not be
+   issued. */
+#define OVX_NOALLOC_SCRATCH_BYTES (4u << 20)
+int ovx_fastconv_exec_noalloc(ovx_plan plan, const void *x, long ldx,
+                              const void *w, long ldw, void *y, long ldy,
+                              long rows, void *stream);
+
 /* ---- vector-matrix multiply -------------------------------------------- */
 /* out = alpha * vmmul<dim>(v, m) over rows x cols, device pointers:
    dim 0: out(r, c) = alpha * v(c) * m(r, c);  dim 1: out(r, c) = alpha *

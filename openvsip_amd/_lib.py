@@ -67,6 +67,7 @@ SIGNATURES = {
     "ovx_fft_in_place": (_I, [_P, _P, _L, _L, _P]),
     "ovx_fastconv_plan_create": (_I, [_PP, _L, _L, _F, _U]),
     "ovx_fastconv_exec": (_I, [_P, _P, _L, _P, _L, _P, _L, _L, _P]),
+    "ovx_fastconv_exec_noalloc": (_I, [_P, _P, _L, _P, _L, _P, _L, _L, _P]),
     "ovx_vmmul": (_I, [_I, _I, _P, _L, _I, _P, _L, _L, _P, _L, _L, _L, _L, _F, _P]),
     "ovx_replica_create": (_I, [_L, _P, _L, _I, ctypes.POINTER(_I)]),
     "ovx_replica_destroy": (_I, [_I]),

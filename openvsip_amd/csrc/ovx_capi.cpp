@@ -620,6 +620,34 @@ static int with_staging(ovx_plan p, const void *in, HostLines gi, void *out, Hos
   return OVX_OK;
 }
 
+// ---------------------------------------------------------------- no-allocation scratch
+// The scratch of ovx_fastconv_exec_noalloc: a fixed device array (each GPU
+// the library runs on holds its own copy until the process ends).  Host code
+// never uses the array's name as an address: its device address is looked up
+// once per device (noalloc_setup, from ovx_init or the entry point's first
+// call on that device) and kept in g_noalloc_ptr.
+static __device__ __align__(256) unsigned char scratch[OVX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+} // namespace
+
+// With `device` current: look up this device's copy of the scratch array.
+static int noalloc_setup(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return fail(OVX_ERR_UNSUPPORTED, "device %d beyond the scratch table", device);
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return OVX_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  if (!ptr) return fail(OVX_ERR_DEVICE, "no device address for the scratch array");
+  g_noalloc_ptr[device] = ptr;
+  return OVX_OK;
+}
+
 // ---------------------------------------------------------------- library
 extern "C" int ovx_init(int device)
 {
@@ -630,7 +658,7 @@ extern "C" int ovx_init(int device)
   e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   g_device = device;
-  return OVX_OK;
+  return noalloc_setup(device);
 }
 
 extern "C" const char *ovx_last_error(void) { return g_err.c_str(); }
@@ -990,6 +1018,39 @@ static int fastconv_exec_body(ovx_plan p, const void *x, long ldx, const void *w
     });
   }
   return p->line.fastconv((const float2 *)x, ldx, (const float2 *)w, ldw, (float2 *)y, ldy, rows, p->scale, s);
+}
+
+// ovx_fastconv_exec with the call's scratch -- the permuted spectrum of a
+// hierarchical or four-step plan -- in the library's fixed device array.
+// Nothing is allocated (the plan's tables were made when the plan was
+// created).
+extern "C" int ovx_fastconv_exec_noalloc(ovx_plan p, const void *x, long ldx, const void *w, long ldw, void *y,
+                                         long ldy, long rows, void *stream)
+{
+  int st = check_plan(p, PLAN_FASTCONV);
+  if (st) return st;
+  if (p->flags & OVX_FLAG_HOST_POINTERS)
+    return fail(OVX_ERR_UNSUPPORTED, "the no-allocation fastconv takes device pointers (staging would allocate)");
+  if (rows < 0 || rows > p->rows) return fail(OVX_ERR_INVALID, "rows %ld outside the plan's %ld", rows, p->rows);
+  const long n = p->cols;
+  if (rows > 1 && (ldx < n || ldy < n)) return fail(OVX_ERR_INVALID, "row stride shorter than n");
+  if (ldw != 0 && rows > 1 && ldw < n) return fail(OVX_ERR_INVALID, "coefficient row stride shorter than n");
+  if (rows == 0) return OVX_OK;
+  if (!x || !w || !y) return fail(OVX_ERR_INVALID, "null data pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->bf16x3) return fastconv_exec_body(p, x, ldx, w, ldw, y, ldy, rows, s); // reads no scratch
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if ((st = noalloc_setup(dev))) return st;
+  const size_t wp_bytes = p->line.fastconv_scratch_wp(rows, ldw) * sizeof(float2);
+  if (wp_bytes > OVX_NOALLOC_SCRATCH_BYTES)
+    return fail(OVX_ERR_UNSUPPORTED, "the call needs %zu bytes of scratch; the no-allocation limit is %u", wp_bytes,
+                (unsigned)OVX_NOALLOC_SCRATCH_BYTES);
+  ExtScratch ext;
+  ext.wp = (float2 *)g_noalloc_ptr[dev];
+  ext.wp_elems = OVX_NOALLOC_SCRATCH_BYTES / sizeof(float2);
+  return p->line.fastconv((const float2 *)x, ldx, (const float2 *)w, ldw, (float2 *)y, ldy, rows, p->scale, s, &ext);
 }
 
 // ---------------------------------------------------------------- split complex
