@@ -114,7 +114,15 @@ __device__ __forceinline__ void exchange_sync()
 }
 
 // ---------------------------------------------------------------- engine
-template <int N, int P, bool WAVE = false>
+// PREF (the default): issue each stage's twiddle loads during the previous
+// stage's LDS exchange (after its writes, before the synchronisation), into
+// registers the exchanged values have just left, so that the stage finds them
+// loaded.  PREF = false loads each twiddle just before its multiply and
+// leaves it to the scheduler to batch the loads, which it does not do in
+// kernels whose register budget is tight: it then waits on every load in
+// turn (the four-step passes and the batch tiles did, 15 to 80 times per
+// transform).
+template <int N, int P, bool WAVE = false, bool PREF = true>
 struct Engine {
   static constexpr int T = N / P;
   static constexpr RadixPlan plan = make_plan(N, P);
@@ -149,10 +157,32 @@ struct Engine {
     return true;
   }
 
+  // The twiddles of stage S for this thread's M = P / R butterflies,
+  // w[m (R-1) + r - 1] = tw[twoff_s + (r-1) Ns + (t + m T) % Ns].
+  template <int S>
+  __device__ __forceinline__ static void load_tw(int t, rsrc_t tw, float2 (&w)[P])
+  {
+    if constexpr (S < NST) {
+      constexpr int R = plan.radix[S];
+      constexpr int Ns = plan.ns[S];
+      constexpr int M = P / R;
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int k = (t + m * T) % Ns;
+#pragma unroll
+          for (int r = 1; r < R; ++r)
+            w[m * (R - 1) + r - 1] = bload(tw, (uint32_t)k * 8u, (uint32_t)(plan.twoff[S] + (r - 1) * Ns) * 8u);
+        }
+      }
+    }
+  }
+
   // One stage.  tw: concatenated forward twiddle table (FP32 values of FP64
-  // W_{Ns R}^{r k}, layout tw[twoff_s + (r-1)*Ns + k]).
+  // W_{Ns R}^{r k}, layout tw[twoff_s + (r-1)*Ns + k]); w: this stage's
+  // twiddles when PREF (loaded by the previous stage), unused otherwise.
   template <int S, int DIR, typename LP>
-  __device__ __forceinline__ static void stage(float2 (&v)[P], LP lds, int t, rsrc_t tw)
+  __device__ __forceinline__ static void stage(float2 (&v)[P], LP lds, int t, rsrc_t tw, float2 (&w)[P])
   {
     constexpr int R = plan.radix[S];
     constexpr int Ns = plan.ns[S];
@@ -168,8 +198,12 @@ struct Engine {
         const int k = j % Ns;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = bload(tw, (uint32_t)k * 8u, (uint32_t)(plan.twoff[S] + (r - 1) * Ns) * 8u);
-          a[r] = twmul<DIR>(a[r], w);
+          if constexpr (PREF) {
+            a[r] = twmul<DIR>(a[r], w[m * (R - 1) + r - 1]);
+          } else {
+            float2 wr = bload(tw, (uint32_t)k * 8u, (uint32_t)(plan.twoff[S] + (r - 1) * Ns) * 8u);
+            a[r] = twmul<DIR>(a[r], wr);
+          }
         }
       }
       Dft<R, DIR>::run(a);
@@ -189,6 +223,7 @@ struct Engine {
       }
     }
     if constexpr (!last) {
+      if constexpr (PREF) load_tw<S + 1>(t, tw, w);
       exchange_sync<WAVE>();
       if constexpr (read_split()) {
         const int b = Layout::idx(t);
@@ -202,12 +237,21 @@ struct Engine {
     }
   }
 
+  // One stage on its own (a caller that runs a single stage).
   template <int S, int DIR, typename LP>
-  __device__ __forceinline__ static void stages_from(float2 (&v)[P], LP lds, int t, rsrc_t tw)
+  __device__ __forceinline__ static void stage(float2 (&v)[P], LP lds, int t, rsrc_t tw)
+  {
+    float2 w[P];
+    if constexpr (PREF) load_tw<S>(t, tw, w);
+    stage<S, DIR>(v, lds, t, tw, w);
+  }
+
+  template <int S, int DIR, typename LP>
+  __device__ __forceinline__ static void stages_from(float2 (&v)[P], LP lds, int t, rsrc_t tw, float2 (&w)[P])
   {
     if constexpr (S < NST) {
-      stage<S, DIR>(v, lds, t, tw);
-      stages_from<S + 1, DIR>(v, lds, t, tw);
+      stage<S, DIR>(v, lds, t, tw, w);
+      stages_from<S + 1, DIR>(v, lds, t, tw, w);
     }
   }
 
@@ -217,7 +261,9 @@ struct Engine {
   template <int DIR, typename LP>
   __device__ __forceinline__ static void fft(float2 (&v)[P], LP lds, int t, rsrc_t tw)
   {
-    stages_from<0, DIR>(v, lds, t, tw);
+    float2 w[P];
+    if constexpr (PREF) load_tw<0>(t, tw, w); // stage 0 has Ns = 1: nothing
+    stages_from<0, DIR>(v, lds, t, tw, w);
   }
 };
 

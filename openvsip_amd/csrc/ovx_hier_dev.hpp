@@ -49,6 +49,14 @@ __device__ __forceinline__ void outer_forward(const IO &ix, bool valid, LP lds, 
 {
   using H = Hier<N>;
   constexpr int R0 = H::R0, M0 = H::M0, NW = H::NW;
+  // the twiddles of all M0 butterflies are loaded up front, with the data,
+  // so that their loads overlap instead of each waiting in turn
+  float2 tw[M0 * R0];
+#pragma unroll
+  for (int m = 0; m < M0; ++m)
+#pragma unroll
+    for (int k1 = 1; k1 < R0; ++k1)
+      tw[m * R0 + k1] = bload(rto, (uint32_t)(t + H::TR * m) * 8u, (uint32_t)((k1 - 1) * NW) * 8u);
 #pragma unroll
   for (int m = 0; m < M0; ++m) {
     const int j = t + H::TR * m;
@@ -59,7 +67,7 @@ __device__ __forceinline__ void outer_forward(const IO &ix, bool valid, LP lds, 
     if constexpr (R0 > 1) {
       Dft<R0, -1>::run(a);
 #pragma unroll
-      for (int k1 = 1; k1 < R0; ++k1) a[k1] = cmul(a[k1], bload(rto, (uint32_t)j * 8u, (uint32_t)((k1 - 1) * NW) * 8u));
+      for (int k1 = 1; k1 < R0; ++k1) a[k1] = cmul(a[k1], tw[m * R0 + k1]);
     }
 #pragma unroll
     for (int k1 = 0; k1 < R0; ++k1) lds_st(lds, k1 * H::LSL + H::WE::Layout::idx(j), a[k1]);
