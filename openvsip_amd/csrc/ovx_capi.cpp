@@ -951,8 +951,12 @@ extern "C" int ovx_fastconv_plan_create(ovx_plan *plan, long rows, long n, float
 }
 
 // A plan's device scratch is shared by its calls, which are ordered on one
-// stream: a call on another stream while the last call may still be running
-// is refused (OVX_ERR_STATE) instead of racing on that scratch.
+// stream: a call that uses the scratch on another stream while the last such
+// call may still be running is refused (OVX_ERR_STATE) instead of racing on
+// it.  Calls that use no plan scratch (a prepared replica spectrum, the
+// register-Stockham and BF16x3 kernels reading the spectrum as given) skip
+// the check and its event record, which costs as much as a few microseconds
+// between back-to-back launches.
 static int claim_plan(ovx_plan p, hipStream_t s)
 {
   if (!p->busy || p->busy_stream == s) return OVX_OK;
@@ -984,6 +988,10 @@ extern "C" int ovx_fastconv_exec(ovx_plan p, const void *x, long ldx, const void
   if (st) return st;
   hipStream_t s = (hipStream_t)stream;
   if (p->flags & OVX_FLAG_HOST_POINTERS) return fastconv_exec_body(p, x, ldx, w, ldw, y, ldy, rows, s);
+  // the hierarchical and four-step kernels read the spectrum permuted into
+  // the plan's buffer
+  const bool scratch = !p->bf16x3 && (p->line.kind == LineFFT::HIER || p->line.kind == LineFFT::FOURSTEP);
+  if (!scratch) return fastconv_exec_body(p, x, ldx, w, ldw, y, ldy, rows, s);
   if ((st = claim_plan(p, s))) return st;
   return mark_plan(p, s, fastconv_exec_body(p, x, ldx, w, ldw, y, ldy, rows, s));
 }
@@ -1415,10 +1423,10 @@ extern "C" int ovx_fastconv_exec_replica(ovx_plan p, const void *x, long ldx, in
   hipStream_t s = (hipStream_t)stream;
   if (rows == 0) return OVX_OK;
   if (p->bf16x3) return ovx_fastconv_exec(p, x, ldx, spec, 0, y, ldy, rows, stream);
-  if ((st = claim_plan(p, s))) return st;
+  // the prepared spectrum is the replica's: no plan scratch
   if (p->scale == 1.0f / (float)n && replica_prepared(replica, &prep, &rn, &p->line) == OVX_OK)
-    return mark_plan(p, s, p->line.fastconv_prepared((const float2 *)x, ldx, prep, (float2 *)y, ldy, rows, s));
-  return mark_plan(p, s, p->line.fastconv((const float2 *)x, ldx, spec, 0, (float2 *)y, ldy, rows, p->scale, s));
+    return p->line.fastconv_prepared((const float2 *)x, ldx, prep, (float2 *)y, ldy, rows, s);
+  return ovx_fastconv_exec(p, x, ldx, spec, 0, y, ldy, rows, stream);
 }
 
 // ---------------------------------------------------------------- plans
