@@ -97,7 +97,6 @@ using namespace ovx;
 ovx_plan_s::~ovx_plan_s()
 {
   delete corr;
-  if (busy) (void)hipEventDestroy(busy);
 }
 
 const char *ovx_plan_s::name() const

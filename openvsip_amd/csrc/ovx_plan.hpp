@@ -112,11 +112,10 @@ struct ovx_plan_s {
   ovx::DevBuf u256;    // its 256th roots of unity
   ovx::DevBuf wtab;    // real Fftm: exp(-2 pi i k / n), k <= n/4
   ovx::DevBuf split_ws; // split-complex calls: interleaved copy of the lines
-  // the stream of the plan's last device-pointer fastconv and an event
-  // recorded after it (its scratch -- the permuted spectrum -- is shared by
-  // the plan's calls)
-  hipEvent_t busy = nullptr;
-  hipStream_t busy_stream = nullptr;
+  // the stream of the plan's last device-pointer call that used its scratch
+  // (the permuted spectrum, shared by the plan's calls), once there was one
+  bool scratch_used = false;
+  hipStream_t scratch_stream = nullptr;
 
   ~ovx_plan_s();
   const char *name() const;
