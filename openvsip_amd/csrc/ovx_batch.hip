@@ -234,16 +234,21 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restr
     }
     __syncthreads(); // LDS reuse by the next tile
   }
+  // The last workgroup to leave resets the tile counter and the exit count
+  // for the next launch: every workgroup's last counter fetch precedes its
+  // exit increment, so no fetch of this launch can follow the reset.
+  if (threadIdx.x == 0) {
+    const int left = atomicAdd(counter + 1, 1);
+    if (left == (int)gridDim.x - 1) {
+      atomicExch(counter, 0);
+      atomicExch(counter + 1, 0);
+    }
+  }
 }
-
-__global__ void k_reset_counter(int *counter) { *counter = 0; }
 
 hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int *counter, int grid,
                         hipStream_t s)
 {
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, counter);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBatchThreads), 0, s, jobs, tiles, ntiles, counter);
   return hipGetLastError();
 }

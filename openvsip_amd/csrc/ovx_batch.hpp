@@ -69,7 +69,9 @@ int batch_fs_blocks_per_cu(int n_line);   // co-resident workgroups of a four-st
 bool batch_supported(int path, int n);
 
 // One persistent launch over tiles[0 .. *ntiles): `grid` workgroups pull
-// tiles with an atomic counter (which this launch resets first).
+// tiles with the atomic counter counter[0]; counter[1] counts the workgroups
+// that have left, and the last one resets both to 0 for the next launch
+// (both must be 0 before the first).
 hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int *counter, int grid,
                         hipStream_t s);
 // Four-step pass `pass` (0, 1, 2) over tiles[0 .. *ntiles) of line length

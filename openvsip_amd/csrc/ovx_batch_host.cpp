@@ -218,11 +218,11 @@ int GraphBatch::build(const std::vector<BatchJob> &jobs, const std::vector<TileQ
   for (const auto &q : queues) nt += q.tiles.size();
   int st;
   if ((st = d_jobs.alloc(std::max<size_t>(jobs.size(), 1) * sizeof(BatchJob))) ||
-      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc((nq + 1) * sizeof(int))))
+      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc((nq + 2) * sizeof(int))))
     return st;
   std::vector<BatchTile> all;
   all.reserve(nt);
-  std::vector<int> counts(nq + 1, 0); // tile counts, then the persistent launch's counter
+  std::vector<int> counts(nq + 2, 0); // tile counts, then the persistent launch's two counters
   std::vector<size_t> offset(nq);
   nlaunch = 0;
   for (int k = 0; k < nq; ++k) {

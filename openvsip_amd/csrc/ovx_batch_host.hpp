@@ -39,8 +39,8 @@ hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[k
 // Prepared spectrum of a resident replica in the layout of path `f`.
 int replica_prepared(int id, const float2 **prep, long *n, const LineFFT *f);
 
-// Server slot tables: tile counts [0 .. kBatchQueues), persistent-launch
-// counter at kMetaCounter.
+// Server slot tables: tile counts [0 .. kBatchQueues), the persistent
+// launch's two counters at kMetaCounter and kMetaCounter + 1.
 constexpr int kMetaInts = 16;
 constexpr int kMetaCounter = 8;
 
