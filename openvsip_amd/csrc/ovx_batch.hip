@@ -4,8 +4,8 @@
 // into tiles of kBatchThreads threads' worth of work: a few whole rows of an
 // LDS-resident job, or a few column/line groups of one row of a four-step
 // job.  A fixed grid of workgroups (one per CU: the LDS footprint is sized for
-// the largest tile) pulls tile indices from an atomic counter until the queue
-// is empty, and runs the same device bodies as the standalone kernels, so a
+// the largest tile) strides over the queue, workgroup w taking tiles w,
+// w + grid, ..., and runs the same device bodies as the standalone kernels, so a
 // job's result is bitwise the same alone and in a batch.  Four-step jobs need
 // their three passes in order; the host splits a batch into up to three
 // queues (pass 1 + LDS-resident work, pass 2, pass 3) launched back to back
@@ -154,23 +154,18 @@ __device__ __forceinline__ FsArgs fs_args_of(const J &j)
 
 __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restrict__ jobs,
                                                           const BatchTile *__restrict__ tiles,
-                                                          const int *__restrict__ ntiles, int *counter)
+                                                          const int *__restrict__ ntiles)
 {
   __shared__ float2 lds_mem[kBatchLds];
   __shared__ BatchJob s_job;
-  __shared__ int s_tile;
   const lds_ptr lds = as_lds(lds_mem);
   lds_job &j = *(lds_job *)&s_job;
   static_assert(sizeof(BatchJob) % 4 == 0 && sizeof(BatchJob) / 4 <= kBatchThreads, "job copy");
   const int total = *ntiles;
-  for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
-    __syncthreads();
-    // uniform across the workgroup: keep it (and the job loaded through it)
-    // in scalar registers
-    const int ti = __builtin_amdgcn_readfirstlane(s_tile);
-    __syncthreads(); // every thread has read s_tile before it is overwritten
-    if (ti >= total) break;
+  // workgroup w takes tiles w, w + grid, ...: the queue is ordered largest
+  // jobs first, so each workgroup gets a like mix (a shared atomic counter
+  // cost each workgroup a cross-XCD round trip per tile)
+  for (int ti = blockIdx.x; ti < total; ti += gridDim.x) {
     const BatchTile tile = tiles[ti];
     if (threadIdx.x < sizeof(BatchJob) / 4)
       ((int *)&s_job)[threadIdx.x] = ((const int *)&jobs[tile.job])[threadIdx.x];
@@ -234,22 +229,11 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restr
     }
     __syncthreads(); // LDS reuse by the next tile
   }
-  // The last workgroup to leave resets the tile counter and the exit count
-  // for the next launch: every workgroup's last counter fetch precedes its
-  // exit increment, so no fetch of this launch can follow the reset.
-  if (threadIdx.x == 0) {
-    const int left = atomicAdd(counter + 1, 1);
-    if (left == (int)gridDim.x - 1) {
-      atomicExch(counter, 0);
-      atomicExch(counter + 1, 0);
-    }
-  }
 }
 
-hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int *counter, int grid,
-                        hipStream_t s)
+hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBatchThreads), 0, s, jobs, tiles, ntiles, counter);
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBatchThreads), 0, s, jobs, tiles, ntiles);
   return hipGetLastError();
 }
 

@@ -68,12 +68,9 @@ int batch_fs_lines_per_group(int n_line); // C of a four-step group of lines of 
 int batch_fs_blocks_per_cu(int n_line);   // co-resident workgroups of a four-step pass launch
 bool batch_supported(int path, int n);
 
-// One persistent launch over tiles[0 .. *ntiles): `grid` workgroups pull
-// tiles with the atomic counter counter[0]; counter[1] counts the workgroups
-// that have left, and the last one resets both to 0 for the next launch
-// (both must be 0 before the first).
-hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int *counter, int grid,
-                        hipStream_t s);
+// One persistent launch over tiles[0 .. *ntiles): `grid` workgroups, workgroup
+// w taking tiles w, w + grid, ...
+hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid, hipStream_t s);
 // Four-step pass `pass` (0, 1, 2) over tiles[0 .. *ntiles) of line length
 // n_line (256 or 512): `grid` workgroups of one line group each take tiles
 // blockIdx.x, blockIdx.x + grid, ... (no shared counter).

@@ -186,7 +186,7 @@ hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[k
                                   const int *counts, bool all, hipStream_t cap)
 {
   hipError_t e = hipSuccess;
-  if (all || counts[0] > 0) e = launch_batch(jobs, tiles[0], meta, meta + kMetaCounter, device_cu_count(), cap);
+  if (all || counts[0] > 0) e = launch_batch(jobs, tiles[0], meta, device_cu_count(), cap);
   for (int k = 1; k < kBatchQueues && e == hipSuccess; ++k) {
     if (!all && counts[k] == 0) continue;
     const int line = fs_queue_line(k);
@@ -218,11 +218,11 @@ int GraphBatch::build(const std::vector<BatchJob> &jobs, const std::vector<TileQ
   for (const auto &q : queues) nt += q.tiles.size();
   int st;
   if ((st = d_jobs.alloc(std::max<size_t>(jobs.size(), 1) * sizeof(BatchJob))) ||
-      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc((nq + 2) * sizeof(int))))
+      (st = d_tiles.alloc(std::max<size_t>(nt, 1) * sizeof(BatchTile))) || (st = d_meta.alloc(std::max(nq, 1) * sizeof(int))))
     return st;
   std::vector<BatchTile> all;
   all.reserve(nt);
-  std::vector<int> counts(nq + 2, 0); // tile counts, then the persistent launch's two counters
+  std::vector<int> counts((size_t)std::max(nq, 1), 0); // tile counts
   std::vector<size_t> offset(nq);
   nlaunch = 0;
   for (int k = 0; k < nq; ++k) {
@@ -251,7 +251,7 @@ int GraphBatch::build(const std::vector<BatchJob> &jobs, const std::vector<TileQ
     for (int k = 0; k < nq && e == hipSuccess; ++k) {
       if (!counts[k]) continue;
       const BatchTile *t = (const BatchTile *)d_tiles.p + offset[k];
-      if (queues[k].line == 0) e = launch_batch(dj, t, meta + k, meta + nq, device_cu_count(), cap);
+      if (queues[k].line == 0) e = launch_batch(dj, t, meta + k, device_cu_count(), cap);
       else e = launch_batch_fs(queues[k].line, queues[k].pass, dj, t, meta + k, counts[k], cap); // one workgroup per tile
     }
     hipError_t e2 = hipStreamEndCapture(cap, &graph);

@@ -39,10 +39,8 @@ hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[k
 // Prepared spectrum of a resident replica in the layout of path `f`.
 int replica_prepared(int id, const float2 **prep, long *n, const LineFFT *f);
 
-// Server slot tables: tile counts [0 .. kBatchQueues), the persistent
-// launch's two counters at kMetaCounter and kMetaCounter + 1.
+// Server slot tables: tile counts [0 .. kBatchQueues).
 constexpr int kMetaInts = 16;
-constexpr int kMetaCounter = 8;
 
 // One launch of a batch: the persistent kernel over LDS-resident tiles
 // (line == 0) or four-step pass `pass` over lines of length `line`.
