@@ -11,7 +11,10 @@
  * the given hipStream_t (passed as void*; NULL = the default stream) without
  * blocking the host.  A plan is not safe for concurrent calls (as the
  * reference's Fft/Fftm objects, which own their plans and buffers); separate
- * plans are independent.
+ * plans are independent.  A device-pointer fastconv call that writes the
+ * plan's scratch (the spectrum permuted for a hierarchical or four-step
+ * kernel) on another stream than the plan's previous such call is refused
+ * (OVX_ERR_STATE) while that stream still has work queued.
  *
  * Reference interfaces replaced (paths relative to the OpenVSIP tree):
  *   fft_fwd / fft_inv            src/ovxx/signal/fft/backend.hpp:19-22
