@@ -114,15 +114,14 @@ __device__ __forceinline__ void exchange_sync()
 }
 
 // ---------------------------------------------------------------- engine
-// PREF (the default): issue each stage's twiddle loads during the previous
-// stage's LDS exchange (after its writes, before the synchronisation), into
-// registers the exchanged values have just left, so that the stage finds them
-// loaded.  PREF = false loads each twiddle just before its multiply and
-// leaves it to the scheduler to batch the loads, which it does not do in
-// kernels whose register budget is tight: it then waits on every load in
-// turn (the four-step passes and the batch tiles did, 15 to 80 times per
-// transform).
-template <int N, int P, bool WAVE = false, bool PREF = true>
+// Each stage's twiddles are loaded during the previous stage's LDS exchange
+// (after its writes, before the synchronisation), into registers the
+// exchanged values have just left, so that the stage finds them loaded.
+// Loaded just before each multiply instead, they were left to the scheduler
+// to batch, which it does not do in kernels whose register budget is tight:
+// it then waited on every load in turn (the four-step passes and the batch
+// tiles did, 15 to 80 times per transform).
+template <int N, int P, bool WAVE = false>
 struct Engine {
   static constexpr int T = N / P;
   static constexpr RadixPlan plan = make_plan(N, P);
@@ -180,7 +179,7 @@ struct Engine {
 
   // One stage.  tw: concatenated forward twiddle table (FP32 values of FP64
   // W_{Ns R}^{r k}, layout tw[twoff_s + (r-1)*Ns + k]); w: this stage's
-  // twiddles when PREF (loaded by the previous stage), unused otherwise.
+  // twiddles (loaded by the previous stage, or by the caller for stage 0).
   template <int S, int DIR, typename LP>
   __device__ __forceinline__ static void stage(float2 (&v)[P], LP lds, int t, rsrc_t tw, float2 (&w)[P])
   {
@@ -195,16 +194,8 @@ struct Engine {
       for (int r = 0; r < R; ++r) a[r] = v[m + r * M];
       const int j = t + m * T;
       if constexpr (Ns > 1) {
-        const int k = j % Ns;
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          if constexpr (PREF) {
-            a[r] = twmul<DIR>(a[r], w[m * (R - 1) + r - 1]);
-          } else {
-            float2 wr = bload(tw, (uint32_t)k * 8u, (uint32_t)(plan.twoff[S] + (r - 1) * Ns) * 8u);
-            a[r] = twmul<DIR>(a[r], wr);
-          }
-        }
+        for (int r = 1; r < R; ++r) a[r] = twmul<DIR>(a[r], w[m * (R - 1) + r - 1]);
       }
       Dft<R, DIR>::run(a);
       if constexpr (last) {
@@ -223,7 +214,7 @@ struct Engine {
       }
     }
     if constexpr (!last) {
-      if constexpr (PREF) load_tw<S + 1>(t, tw, w);
+      load_tw<S + 1>(t, tw, w);
       exchange_sync<WAVE>();
       if constexpr (read_split()) {
         const int b = Layout::idx(t);
@@ -242,7 +233,7 @@ struct Engine {
   __device__ __forceinline__ static void stage(float2 (&v)[P], LP lds, int t, rsrc_t tw)
   {
     float2 w[P];
-    if constexpr (PREF) load_tw<S>(t, tw, w);
+    load_tw<S>(t, tw, w);
     stage<S, DIR>(v, lds, t, tw, w);
   }
 
@@ -262,7 +253,7 @@ struct Engine {
   __device__ __forceinline__ static void fft(float2 (&v)[P], LP lds, int t, rsrc_t tw)
   {
     float2 w[P];
-    if constexpr (PREF) load_tw<0>(t, tw, w); // stage 0 has Ns = 1: nothing
+    load_tw<0>(t, tw, w); // stage 0 has Ns = 1: nothing
     stages_from<0, DIR>(v, lds, t, tw, w);
   }
 };
