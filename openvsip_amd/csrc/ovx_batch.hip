@@ -162,9 +162,10 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restr
   lds_job &j = *(lds_job *)&s_job;
   static_assert(sizeof(BatchJob) % 4 == 0 && sizeof(BatchJob) / 4 <= kBatchThreads, "job copy");
   const int total = *ntiles;
-  // workgroup w takes tiles w, w + grid, ...: the queue is ordered largest
-  // jobs first, so each workgroup gets a like mix (a shared atomic counter
-  // cost each workgroup a cross-XCD round trip per tile)
+  // workgroup w takes tiles w, w + grid, ...: consecutive tiles (rows of one
+  // job, jobs of like size next to each other) go to different workgroups,
+  // so each gets a like share of the queue (a shared atomic counter cost
+  // each workgroup a cross-XCD round trip per tile)
   for (int ti = blockIdx.x; ti < total; ti += gridDim.x) {
     const BatchTile tile = tiles[ti];
     if (threadIdx.x < sizeof(BatchJob) / 4)
