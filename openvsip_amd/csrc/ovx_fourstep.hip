@@ -52,18 +52,31 @@ __global__ void k_fs_permute(const float2 *__restrict__ w, long ldw, float2 *__r
 
 // ---------------------------------------------------------------- dispatch
 
+// Passes that read or write split storage move 4-byte values, so their
+// groups take 32 lines where the thread count allows (128-byte runs per
+// array instead of 64).
+template <int L>
+struct FsSplitCfg {
+  static constexpr int P = FsCfg<L>::P;
+  static constexpr int C = (L / P) * 32 <= 1024 ? 32 : FsCfg<L>::C;
+  static constexpr int NT = C * (L / P);
+};
+
 template <int L, int MODE>
 static hipError_t col_launch(const FsArgs &a, long rows, hipStream_t s)
 {
   using F = FsCfg<L>;
-  dim3 grid((unsigned)(a.n2 / F::C), (unsigned)rows);
+  using G = FsSplitCfg<L>;
   // split storage is read by a call's first pass or written by its last
   if (a.in_re)
-    hipLaunchKernelGGL((k_fs_col<L, F::P, F::C, MODE, true, false>), grid, dim3(F::NT), 0, s, a);
+    hipLaunchKernelGGL((k_fs_col<L, G::P, G::C, MODE, true, false>), dim3((unsigned)(a.n2 / G::C), (unsigned)rows),
+                       dim3(G::NT), 0, s, a);
   else if (a.out_re)
-    hipLaunchKernelGGL((k_fs_col<L, F::P, F::C, MODE, false, true>), grid, dim3(F::NT), 0, s, a);
+    hipLaunchKernelGGL((k_fs_col<L, G::P, G::C, MODE, false, true>), dim3((unsigned)(a.n2 / G::C), (unsigned)rows),
+                       dim3(G::NT), 0, s, a);
   else
-    hipLaunchKernelGGL((k_fs_col<L, F::P, F::C, MODE, false, false>), grid, dim3(F::NT), 0, s, a);
+    hipLaunchKernelGGL((k_fs_col<L, F::P, F::C, MODE, false, false>), dim3((unsigned)(a.n2 / F::C), (unsigned)rows),
+                       dim3(F::NT), 0, s, a);
   return hipGetLastError();
 }
 
@@ -71,11 +84,13 @@ template <int L, int MODE>
 static hipError_t row_launch(const FsArgs &a, long rows, hipStream_t s)
 {
   using F = FsCfg<L>;
-  dim3 grid((unsigned)(a.n1 / F::C), (unsigned)rows);
+  using G = FsSplitCfg<L>;
   if (a.out_re)
-    hipLaunchKernelGGL((k_fs_row<L, F::P, F::C, MODE, true>), grid, dim3(F::NT), 0, s, a);
+    hipLaunchKernelGGL((k_fs_row<L, G::P, G::C, MODE, true>), dim3((unsigned)(a.n1 / G::C), (unsigned)rows),
+                       dim3(G::NT), 0, s, a);
   else
-    hipLaunchKernelGGL((k_fs_row<L, F::P, F::C, MODE, false>), grid, dim3(F::NT), 0, s, a);
+    hipLaunchKernelGGL((k_fs_row<L, F::P, F::C, MODE, false>), dim3((unsigned)(a.n1 / F::C), (unsigned)rows),
+                       dim3(F::NT), 0, s, a);
   return hipGetLastError();
 }
 
