@@ -112,7 +112,7 @@ int ovx_autotune_fastconv(long n, long rows, int *best_variant, double *best_us)
 /* Cached variant for (n, rows), or -1. */
 int ovx_tuned_variant(long n, long rows);
 /* Batch tiles (no reference counterpart): for a job of rows x n inside a
-   batch or the server, time every tile body the persistent kernel holds for
+   batch or the server, time every tile body the persistent kernels hold for
    length n (the hierarchical body and the register-Stockham body with the
    batch's values per thread) as a one-job batch replayed from its graph, and
    record the fastest under (n, rows rounded up to a power of two) in the
@@ -324,12 +324,14 @@ int ovx_storage_info(ovx_storage storage, unsigned *valid, uint64_t *h2d_bytes,
 /* ---- batches of uneven fastconv jobs ------------------------------------ */
 /* No reference counterpart: the reference runs one Fftm object per shape
    (benchmarks/fastconv.cpp loops one size at a time).  A batch packs jobs of
-   different n and row counts (each against a resident replica) into one
-   persistent-kernel launch over an (instance, rows) tile queue; the rows of
-   jobs on the four-step path run in chunks of about 128 MiB, each chunk one
-   launch per pass and line length (one line group per workgroup), so that a
-   chunk stays in the Infinity Cache between its passes.  The launches are captured once into a hipGraph at commit
-   and replayed by every exec.  Each job's result is bitwise the same as
+   different n and row counts (each against a resident replica) into
+   persistent-kernel launches over (instance, rows) tile queues (one of
+   1024-thread workgroups for rows of 12288 and 16384 points, one of
+   512-thread workgroups for shorter rows); the rows of jobs on the
+   four-step path run in chunks of about 128 MiB, each chunk one launch per
+   pass and line length (one line group per workgroup), so that a chunk
+   stays in the Infinity Cache between its passes.  The launches are
+   captured once into a hipGraph at commit and replayed by every exec.  Each job's result is bitwise the same as
    ovx_fastconv_exec_replica on the same job.  ovx_batch_launches returns
    the number of kernel launches per replay.  A committed batch holds the
    device addresses of its jobs' buffers and replica spectra: they must stay

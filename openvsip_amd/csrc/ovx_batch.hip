@@ -1,15 +1,16 @@
-// ovx_batch.hip -- one persistent launch over an (instance, rows) tile queue.
+// ovx_batch.hip -- persistent launches over (instance, rows) tile queues.
 //
 // A batch of uneven fast-convolution jobs (different N and row counts) is cut
-// into tiles of kBatchThreads threads' worth of work: a few whole rows of an
-// LDS-resident job, or a few column/line groups of one row of a four-step
-// job.  A fixed grid of workgroups (one per CU: the LDS footprint is sized for
-// the largest tile) strides over the queue, workgroup w taking tiles w,
-// w + grid, ..., and runs the same device bodies as the standalone kernels, so a
-// job's result is bitwise the same alone and in a batch.  Four-step jobs need
-// their three passes in order; the host splits a batch into up to three
-// queues (pass 1 + LDS-resident work, pass 2, pass 3) launched back to back
-// in one captured hipGraph.
+// into tiles: a few whole rows of an LDS-resident job (one workgroup's worth
+// of threads), or one column/line group of one row of a four-step job.  The
+// LDS-resident tiles run in two persistent launches, one of 1024-thread
+// workgroups (one per CU; rows of 12288 and 16384 points) and one of
+// 512-thread workgroups (two per CU; every shorter row), whose fixed grids
+// stride over their queues, workgroup w taking tiles w, w + grid, ...  Every
+// tile runs the same device body as the standalone kernels, so a job's
+// result is bitwise the same alone and in a batch.  Four-step jobs need their
+// three passes in order; each pass is a launch of its own over its queue, and
+// the host captures all of a batch's launches in one hipGraph.
 #include "ovx_batch.hpp"
 #include "ovx_fs_dev.hpp"
 #include "ovx_hier_dev.hpp"
@@ -18,10 +19,18 @@
 
 namespace ovx {
 
-// LDS of the largest tile (1024 threads): 64 register-Stockham rows of 256
-// (17408 float2), four-step groups 4 x 16 x 273; every tile function checks
-// its need against it.
+// LDS of the largest tile of each persistent launch.  1024 threads: 64
+// register-Stockham rows of 256 (17408 float2), four-step groups
+// 4 x 16 x 273.  512 threads: every tile routed there (rows of up to 8192
+// points, overlap-save segments) needs 16 lines' worth of 512 threads,
+// 16 * 512 * 33 / 32 = 8448 float2 (67.6 KB: two workgroups per CU).  Every
+// tile function checks its need against its launch's figure.
 constexpr int kBatchLds = 4 * 16 * 273 > 17408 ? 4 * 16 * 273 : 17408;
+constexpr int kBatchMidLds = 8448;
+template <int NT>
+struct BatchLds {
+  static constexpr int value = NT == kBatchThreads ? kBatchLds : kBatchMidLds;
+};
 
 // The job descriptor of the current tile lives in LDS (copied once per tile
 // by the persistent loop); the tile functions read it from there.
@@ -44,12 +53,14 @@ __device__ __forceinline__ T uniform(const __attribute__((address_space(3))) T &
   return u;
 }
 
-template <int N>
+// Tile functions of a workgroup of NT threads: NT / (threads of a row) rows
+// (or overlap-save segments) per tile.
+template <int NT, int N>
 __device__ __noinline__ void tile_hier_fastconv(lds_job &j, int row0_arg, lds_ptr lds)
 {
   const int row0 = __builtin_amdgcn_readfirstlane(row0_arg); // uniform, see uniform()
   using H = Hier<N>;
-  static_assert((kBatchThreads / H::TR) * H::LROW <= kBatchLds, "tile LDS");
+  static_assert(NT % H::TR == 0 && (NT / H::TR) * H::LROW <= BatchLds<NT>::value, "tile LDS");
   HierArgs a{};
   a.x = uniform(j.x);
   a.ldx = uniform(j.ldx);
@@ -65,15 +76,15 @@ __device__ __noinline__ void tile_hier_fastconv(lds_job &j, int row0_arg, lds_pt
   const int lrow = (int)threadIdx.x / H::TR;
   const int t = (int)threadIdx.x % H::TR;
   // a tile of one row is only made for a row that exists
-  hier_fastconv_body<N, kBatchThreads / H::TR == 1>(a, row0, lrow, t, lds + lrow * H::LROW);
+  hier_fastconv_body<N, NT / H::TR == 1>(a, row0, lrow, t, lds + lrow * H::LROW);
 }
 
-template <int N>
+template <int NT, int N>
 __device__ __noinline__ void tile_hier_fwd(lds_job &j, int row0_arg, lds_ptr lds)
 {
   const int row0 = __builtin_amdgcn_readfirstlane(row0_arg); // uniform, see uniform()
   using H = Hier<N>;
-  static_assert((kBatchThreads / H::TR) * H::LROW <= kBatchLds, "tile LDS");
+  static_assert(NT % H::TR == 0 && (NT / H::TR) * H::LROW <= BatchLds<NT>::value, "tile LDS");
   HierArgs a{};
   a.x = uniform(j.x);
   a.ldx = uniform(j.ldx);
@@ -88,41 +99,40 @@ __device__ __noinline__ void tile_hier_fwd(lds_job &j, int row0_arg, lds_ptr lds
   hier_fwd_body<N>(a, row0, lrow, t, lds + lrow * H::LROW);
 }
 
-template <int N, int P>
+template <int NT, int N, int P>
 __device__ __noinline__ void tile_row_fastconv(lds_job &j, int row0_arg, lds_ptr lds)
 {
   const int row0 = __builtin_amdgcn_readfirstlane(row0_arg); // uniform, see uniform()
   using E = Engine<N, P, (N / P <= 64)>;
-  static_assert((kBatchThreads / E::T) * E::LROW <= kBatchLds, "tile LDS");
+  static_assert(NT % E::T == 0 && (NT / E::T) * E::LROW <= BatchLds<NT>::value, "tile LDS");
   const int lrow = (int)threadIdx.x / E::T;
   const int t = (int)threadIdx.x % E::T;
   row_fastconv_body<N, P>(uniform(j.x), uniform(j.ldx), uniform(j.w), 0, uniform(j.y), uniform(j.ldy),
                           uniform(j.rows), uniform(j.tw0), 1.0f, row0, lrow, t, lds + lrow * E::LROW);
 }
 
-template <int N, int P>
+template <int NT, int N, int P>
 __device__ __noinline__ void tile_row_fwd(lds_job &j, int row0_arg, lds_ptr lds)
 {
   const int row0 = __builtin_amdgcn_readfirstlane(row0_arg); // uniform, see uniform()
   using E = Engine<N, P, (N / P <= 64)>;
-  static_assert((kBatchThreads / E::T) * E::LROW <= kBatchLds, "tile LDS");
+  static_assert(NT % E::T == 0 && (NT / E::T) * E::LROW <= BatchLds<NT>::value, "tile LDS");
   const int lrow = (int)threadIdx.x / E::T;
   const int t = (int)threadIdx.x % E::T;
   row_fft_body<N, P, -1>(uniform(j.x), uniform(j.ldx), uniform(j.y), uniform(j.ldy), uniform(j.rows),
                          uniform(j.tw0), uniform(j.scale), row0, lrow, t, lds + lrow * E::LROW);
 }
 
-// Overlap-save segments seg0, seg0 + 1, ... (kBatchThreads / (L/16) of them
-// per tile); a copy past the last segment runs with P = 0, which stores
-// nothing, because ols_body's barriers need every thread.
-template <int L>
+// Overlap-save segments seg0, seg0 + 1, ... (NT / (L/16) of them per tile); a
+// copy past the last segment runs with P = 0, which stores nothing, because
+// ols_body's barriers need every thread.
+template <int NT, int L>
 __device__ __noinline__ void tile_ols(lds_job &j, int seg0_arg, lds_ptr lds)
 {
   const int seg0 = __builtin_amdgcn_readfirstlane(seg0_arg); // uniform, see uniform()
   OlsArgs a = uniform(j.ols);
   constexpr int T = L / 16;
-  static_assert(T <= kBatchThreads, "segment too long");
-  static_assert((kBatchThreads / T) * Engine<L, 16>::LROW <= kBatchLds, "tile LDS");
+  static_assert(NT % T == 0 && (NT / T) * Engine<L, 16>::LROW <= BatchLds<NT>::value, "tile LDS");
   const int t = (int)threadIdx.x % T;
   const int copy = (int)threadIdx.x / T;
   const long seg = (long)seg0 + copy;
@@ -131,7 +141,10 @@ __device__ __noinline__ void tile_ols(lds_job &j, int seg0_arg, lds_ptr lds)
   ols_body<L, 16>(a, seg, t, lds + copy * Engine<L, 16>::LROW);
 }
 
-int batch_ols_segments_per_tile(int L) { return kBatchThreads / (L / 16); }
+int batch_ols_segments_per_tile(int L)
+{
+  return (batch_ols_mid_routed(L) ? kBatchMidThreads : kBatchThreads) / (L / 16);
+}
 
 template <typename J>
 __device__ __forceinline__ FsArgs fs_args_of(const J &j)
@@ -152,77 +165,90 @@ __device__ __forceinline__ FsArgs fs_args_of(const J &j)
   return a;
 }
 
-__global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restrict__ jobs,
-                                                          const BatchTile *__restrict__ tiles,
-                                                          const int *__restrict__ ntiles)
+// Whether a tile of (path, n) -- or an overlap-save segment of length L --
+// runs in the launch of NT threads: the host routes each to exactly one of
+// the two (batch_queue, batch_ols_queue), and the other launch's switch has
+// no code for it.
+template <int NT, int PATH, int N>
+constexpr bool runs_in() { return batch_mid_routed(PATH, N) == (NT == kBatchMidThreads); }
+template <int NT, int L>
+constexpr bool ols_runs_in() { return batch_ols_mid_routed(L) == (NT == kBatchMidThreads); }
+
+template <int NT, int N>
+__device__ __forceinline__ void run_hier(bool fwd_only, lds_job &j, int a, lds_ptr lds)
 {
-  __shared__ float2 lds_mem[kBatchLds];
-  __shared__ BatchJob s_job;
+  if constexpr (runs_in<NT, BP_HIER, N>()) {
+    if (fwd_only) tile_hier_fwd<NT, N>(j, a, lds);
+    else tile_hier_fastconv<NT, N>(j, a, lds);
+  }
+}
+template <int NT, int N, int P>
+__device__ __forceinline__ void run_row(bool fwd_only, lds_job &j, int a, lds_ptr lds)
+{
+  if constexpr (runs_in<NT, BP_ROW, N>()) {
+    if (fwd_only) tile_row_fwd<NT, N, P>(j, a, lds);
+    else tile_row_fastconv<NT, N, P>(j, a, lds);
+  }
+}
+template <int NT, int L>
+__device__ __forceinline__ void run_ols(lds_job &j, int a, lds_ptr lds)
+{
+  if constexpr (ols_runs_in<NT, L>()) tile_ols<NT, L>(j, a, lds);
+}
+
+// The persistent loop of a launch of NT threads: workgroup w takes tiles w,
+// w + grid, ...: consecutive tiles (rows of one job, jobs of like size next to
+// each other) go to different workgroups, so each gets a like share of the
+// queue (a shared atomic counter cost each workgroup a cross-XCD round trip
+// per tile).
+template <int NT>
+__device__ __forceinline__ void batch_loop(const BatchJob *__restrict__ jobs, const BatchTile *__restrict__ tiles,
+                                           const int *__restrict__ ntiles, float2 *lds_mem, BatchJob *s_job)
+{
   const lds_ptr lds = as_lds(lds_mem);
-  lds_job &j = *(lds_job *)&s_job;
-  static_assert(sizeof(BatchJob) % 4 == 0 && sizeof(BatchJob) / 4 <= kBatchThreads, "job copy");
+  lds_job &j = *(lds_job *)s_job;
+  static_assert(sizeof(BatchJob) % 4 == 0 && sizeof(BatchJob) / 4 <= NT, "job copy");
   const int total = *ntiles;
-  // workgroup w takes tiles w, w + grid, ...: consecutive tiles (rows of one
-  // job, jobs of like size next to each other) go to different workgroups,
-  // so each gets a like share of the queue (a shared atomic counter cost
-  // each workgroup a cross-XCD round trip per tile)
   for (int ti = blockIdx.x; ti < total; ti += gridDim.x) {
     const BatchTile tile = tiles[ti];
-    if (threadIdx.x < sizeof(BatchJob) / 4)
-      ((int *)&s_job)[threadIdx.x] = ((const int *)&jobs[tile.job])[threadIdx.x];
+    if (threadIdx.x < sizeof(BatchJob) / 4) ((int *)s_job)[threadIdx.x] = ((const int *)&jobs[tile.job])[threadIdx.x];
     __syncthreads();
     switch (tile.op) {
     case BT_HIER_FASTCONV:
+    case BT_HIER_FFT_FWD: {
+      const bool fwd = tile.op == BT_HIER_FFT_FWD;
       switch (j.n) {
-      case 1024: tile_hier_fastconv<1024>(j, tile.a, lds); break;
-      case 2048: tile_hier_fastconv<2048>(j, tile.a, lds); break;
-      case 4096: tile_hier_fastconv<4096>(j, tile.a, lds); break;
-      case 8192: tile_hier_fastconv<8192>(j, tile.a, lds); break;
-      case 16384: tile_hier_fastconv<16384>(j, tile.a, lds); break;
+      case 1024: run_hier<NT, 1024>(fwd, j, tile.a, lds); break;
+      case 2048: run_hier<NT, 2048>(fwd, j, tile.a, lds); break;
+      case 4096: run_hier<NT, 4096>(fwd, j, tile.a, lds); break;
+      case 8192: run_hier<NT, 8192>(fwd, j, tile.a, lds); break;
+      case 16384: run_hier<NT, 16384>(fwd, j, tile.a, lds); break;
       default: break;
       }
       break;
-    case BT_HIER_FFT_FWD:
-      switch (j.n) {
-      case 1024: tile_hier_fwd<1024>(j, tile.a, lds); break;
-      case 2048: tile_hier_fwd<2048>(j, tile.a, lds); break;
-      case 4096: tile_hier_fwd<4096>(j, tile.a, lds); break;
-      case 8192: tile_hier_fwd<8192>(j, tile.a, lds); break;
-      case 16384: tile_hier_fwd<16384>(j, tile.a, lds); break;
-      default: break;
-      }
-      break;
+    }
     case BT_ROW_FASTCONV:
+    case BT_ROW_FFT_FWD: {
+      const bool fwd = tile.op == BT_ROW_FFT_FWD;
       switch (j.n) {
-      case 256: tile_row_fastconv<256, 16>(j, tile.a, lds); break;
-      case 512: tile_row_fastconv<512, 16>(j, tile.a, lds); break;
-      case 2048: tile_row_fastconv<2048, 16>(j, tile.a, lds); break;
-      case 4096: tile_row_fastconv<4096, 16>(j, tile.a, lds); break;
-      case 3072: tile_row_fastconv<3072, 12>(j, tile.a, lds); break;
-      case 6144: tile_row_fastconv<6144, 12>(j, tile.a, lds); break;
-      case 12288: tile_row_fastconv<12288, 12>(j, tile.a, lds); break;
+      case 256: run_row<NT, 256, 16>(fwd, j, tile.a, lds); break;
+      case 512: run_row<NT, 512, 16>(fwd, j, tile.a, lds); break;
+      case 2048: run_row<NT, 2048, 16>(fwd, j, tile.a, lds); break;
+      case 4096: run_row<NT, 4096, 16>(fwd, j, tile.a, lds); break;
+      case 3072: run_row<NT, 3072, 12>(fwd, j, tile.a, lds); break;
+      case 6144: run_row<NT, 6144, 12>(fwd, j, tile.a, lds); break;
+      case 12288: run_row<NT, 12288, 12>(fwd, j, tile.a, lds); break;
       default: break;
       }
       break;
-    case BT_ROW_FFT_FWD:
-      switch (j.n) {
-      case 256: tile_row_fwd<256, 16>(j, tile.a, lds); break;
-      case 512: tile_row_fwd<512, 16>(j, tile.a, lds); break;
-      case 2048: tile_row_fwd<2048, 16>(j, tile.a, lds); break;
-      case 4096: tile_row_fwd<4096, 16>(j, tile.a, lds); break;
-      case 3072: tile_row_fwd<3072, 12>(j, tile.a, lds); break;
-      case 6144: tile_row_fwd<6144, 12>(j, tile.a, lds); break;
-      case 12288: tile_row_fwd<12288, 12>(j, tile.a, lds); break;
-      default: break;
-      }
-      break;
+    }
     case BT_OLS:
       switch (j.ols_len) {
-      case 1024: tile_ols<1024>(j, tile.a, lds); break;
-      case 2048: tile_ols<2048>(j, tile.a, lds); break;
-      case 4096: tile_ols<4096>(j, tile.a, lds); break;
-      case 8192: tile_ols<8192>(j, tile.a, lds); break;
-      case 16384: tile_ols<16384>(j, tile.a, lds); break;
+      case 1024: run_ols<NT, 1024>(j, tile.a, lds); break;
+      case 2048: run_ols<NT, 2048>(j, tile.a, lds); break;
+      case 4096: run_ols<NT, 4096>(j, tile.a, lds); break;
+      case 8192: run_ols<NT, 8192>(j, tile.a, lds); break;
+      case 16384: run_ols<NT, 16384>(j, tile.a, lds); break;
       default: break;
       }
       break;
@@ -232,9 +258,35 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restr
   }
 }
 
+__global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchJob *__restrict__ jobs,
+                                                          const BatchTile *__restrict__ tiles,
+                                                          const int *__restrict__ ntiles)
+{
+  __shared__ float2 lds_mem[kBatchLds];
+  __shared__ BatchJob s_job;
+  batch_loop<kBatchThreads>(jobs, tiles, ntiles, lds_mem, &s_job);
+}
+
+// Two workgroups per CU: four waves per SIMD (the second launch bound is
+// waves per SIMD; the tile functions inherit it, so they fit 128 VGPRs) and
+// 2 x (67.6 KB + the descriptor) of LDS.
+__global__ __launch_bounds__(kBatchMidThreads, kBatchMidBlocksPerCu * kBatchMidThreads / 256) void k_batch_mid(
+    const BatchJob *__restrict__ jobs, const BatchTile *__restrict__ tiles, const int *__restrict__ ntiles)
+{
+  __shared__ float2 lds_mem[kBatchMidLds];
+  __shared__ BatchJob s_job;
+  batch_loop<kBatchMidThreads>(jobs, tiles, ntiles, lds_mem, &s_job);
+}
+
 hipError_t launch_batch(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid, hipStream_t s)
 {
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(kBatchThreads), 0, s, jobs, tiles, ntiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_mid(const BatchJob *jobs, const BatchTile *tiles, const int *ntiles, int grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_mid, dim3(grid), dim3(kBatchMidThreads), 0, s, jobs, tiles, ntiles);
   return hipGetLastError();
 }
 
@@ -327,16 +379,10 @@ bool batch_supported(int path, int n)
   return false;
 }
 
-// Batch tiles of the 3 * 2^k lengths use 12 values per thread, so that 1024
-// threads fit the 128-VGPR budget of a 1024-thread workgroup; the powers of
-// two use 16.
-int batch_row_p(int n) { return (n == 256 || n == 512 || n == 2048 || n == 4096) ? 16 : 12; }
-
 int batch_rows_per_tile(int path, int n)
 {
-  if (path == BP_HIER) return kBatchThreads / (64 * (n / 1024));
-  if (path == BP_ROW) return kBatchThreads / (n / batch_row_p(n));
-  return 1;
+  if (path != BP_HIER && path != BP_ROW) return 1;
+  return (batch_mid_routed(path, n) ? kBatchMidThreads : kBatchThreads) / batch_line_threads(path, n);
 }
 
 int batch_fs_lines_per_group(int n_line)

@@ -121,14 +121,16 @@ int make_batch_job(const LineFFT &f, long rows, const float2 *x, long ldx, float
   return OVX_OK;
 }
 
-// Tiles of job `ji`: LDS-resident rows go to queue 0, the three four-step
-// passes to their (pass, line length) queues, one line group per tile.
+// Tiles of job `ji`: LDS-resident rows go to the queue of their persistent
+// launch (batch_queue), the three four-step passes to their (pass, line
+// length) queues, one line group per tile.
 void make_batch_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatchQueues])
 {
   if (j.path == BP_HIER || j.path == BP_ROW) {
     int rpt = batch_rows_per_tile(j.path, j.n);
     int op = j.path == BP_HIER ? BT_HIER_FASTCONV : BT_ROW_FASTCONV;
-    for (long r = 0; r < j.rows; r += rpt) q[0].push_back(BatchTile{ ji, op, (int)r, 0 });
+    std::vector<BatchTile> &dst = q[batch_queue(j.path, j.n)];
+    for (long r = 0; r < j.rows; r += rpt) dst.push_back(BatchTile{ ji, op, (int)r, 0 });
     return;
   }
   const int gcol = j.n2 / batch_fs_lines_per_group(j.n1), grow = j.n1 / batch_fs_lines_per_group(j.n2);
@@ -147,7 +149,7 @@ std::vector<TileQueue> chunked_fs_queues(const std::vector<BatchJob> &jobs, cons
   std::vector<BatchTile> cq[kBatchQueues];
   size_t used = 0;
   auto flush = [&] {
-    for (int k = 1; k < kBatchQueues; ++k)
+    for (int k = 1; k < kFsQueueEnd; ++k)
       if (!cq[k].empty()) {
         out.push_back(TileQueue{ fs_queue_line(k), fs_queue_pass(k), std::move(cq[k]) });
         cq[k].clear();
@@ -187,7 +189,9 @@ hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[k
 {
   hipError_t e = hipSuccess;
   if (all || counts[0] > 0) e = launch_batch(jobs, tiles[0], meta, device_cu_count(), cap);
-  for (int k = 1; k < kBatchQueues && e == hipSuccess; ++k) {
+  if (e == hipSuccess && (all || counts[kMidQueue] > 0))
+    e = launch_batch_mid(jobs, tiles[kMidQueue], meta + kMidQueue, kBatchMidBlocksPerCu * device_cu_count(), cap);
+  for (int k = 1; k < kFsQueueEnd && e == hipSuccess; ++k) {
     if (!all && counts[k] == 0) continue;
     const int line = fs_queue_line(k);
     // a batch with known counts launches one workgroup per tile (as the
@@ -252,6 +256,8 @@ int GraphBatch::build(const std::vector<BatchJob> &jobs, const std::vector<TileQ
       if (!counts[k]) continue;
       const BatchTile *t = (const BatchTile *)d_tiles.p + offset[k];
       if (queues[k].line == 0) e = launch_batch(dj, t, meta + k, device_cu_count(), cap);
+      else if (queues[k].line == kMidLine)
+        e = launch_batch_mid(dj, t, meta + k, kBatchMidBlocksPerCu * device_cu_count(), cap);
       else e = launch_batch_fs(queues[k].line, queues[k].pass, dj, t, meta + k, counts[k], cap); // one workgroup per tile
     }
     hipError_t e2 = hipStreamEndCapture(cap, &graph);
@@ -360,14 +366,16 @@ extern "C" int ovx_batch_commit(ovx_batch b, void *stream)
   std::vector<int> order(b->djobs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->djobs[a].n > b->djobs[c].n; });
-  // Launch order: the persistent launch over every LDS-resident tile, then
-  // the four-step jobs' rows in Infinity-Cache-sized chunks, one launch per
-  // non-empty (pass, line length) queue of a chunk, in pass order.
+  // Launch order: the two persistent launches over the LDS-resident tiles
+  // (1024-thread rows, then the shorter ones), then the four-step jobs' rows
+  // in Infinity-Cache-sized chunks, one launch per non-empty (pass, line
+  // length) queue of a chunk, in pass order.
   std::vector<BatchTile> q[kBatchQueues];
   for (int i : order)
     if (b->djobs[i].path != BP_FS) make_batch_tiles(b->djobs[i], i, q);
   std::vector<TileQueue> queues;
   if (!q[0].empty()) queues.push_back(TileQueue{ 0, 0, std::move(q[0]) });
+  if (!q[kMidQueue].empty()) queues.push_back(TileQueue{ kMidLine, 0, std::move(q[kMidQueue]) });
   for (auto &fq : chunked_fs_queues(b->djobs, order, kFsChunkBytes)) queues.push_back(std::move(fq));
   int st = b->gb.build(b->djobs, queues, s);
   if (st) return st;

@@ -31,8 +31,8 @@ void make_batch_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatch
 // Four-step forward Fftm tiles (x -> workspace -> y) of job `ji`.
 void make_batch_fft_tiles(const BatchJob &j, int ji, std::vector<BatchTile> q[kBatchQueues]);
 // Capture the launches of a batch into `graph` on stream `cap`: the
-// persistent launch over queue 0, then one launch per four-step queue in pass
-// order; `all` launches every queue (tile counts read from device memory at
+// persistent launches over queue 0 and kMidQueue, then one launch per
+// four-step queue in pass order; `all` launches every queue (tile counts read from device memory at
 // run time), otherwise only the non-empty ones in `counts`.
 hipError_t capture_batch_launches(const BatchJob *jobs, BatchTile *const tiles[kBatchQueues], int *meta,
                                   const int *counts, bool all, hipStream_t cap);
@@ -42,8 +42,10 @@ int replica_prepared(int id, const float2 **prep, long *n, const LineFFT *f);
 // Server slot tables: tile counts [0 .. kBatchQueues).
 constexpr int kMetaInts = 16;
 
-// One launch of a batch: the persistent kernel over LDS-resident tiles
-// (line == 0) or four-step pass `pass` over lines of length `line`.
+// One launch of a batch: a persistent kernel over LDS-resident tiles
+// (line == 0: the 1024-thread one, line == kMidLine: the 512-thread one) or
+// four-step pass `pass` over lines of length `line`.
+constexpr int kMidLine = -1;
 struct TileQueue {
   int line = 0;
   int pass = 0;

@@ -262,8 +262,8 @@ int Server::admit_and_launch(Slot &s, std::vector<Req *> &batch)
       j.ols_len = cr->L;
       jobs.push_back(j);
       int spt = batch_ols_segments_per_tile(cr->L);
-      for (long sgm = 0; sgm < corr_segments(*cr); sgm += spt)
-        q[0].push_back(BatchTile{ (int)i, BT_OLS, (int)sgm, 0 });
+      std::vector<BatchTile> &dst = q[batch_ols_queue(cr->L)];
+      for (long sgm = 0; sgm < corr_segments(*cr); sgm += spt) dst.push_back(BatchTile{ (int)i, BT_OLS, (int)sgm, 0 });
       continue;
     }
     const LineFFT *f = shared_path(r.n, r.rows, &st);
@@ -285,7 +285,8 @@ int Server::admit_and_launch(Slot &s, std::vector<Req *> &batch)
       } else {
         int rpt = batch_rows_per_tile(j.path, j.n);
         int op = j.path == BP_HIER ? BT_HIER_FFT_FWD : BT_ROW_FFT_FWD;
-        for (long rr = 0; rr < j.rows; rr += rpt) q[0].push_back(BatchTile{ (int)i, op, (int)rr, 0 });
+        std::vector<BatchTile> &dst = q[batch_queue(j.path, j.n)];
+        for (long rr = 0; rr < j.rows; rr += rpt) dst.push_back(BatchTile{ (int)i, op, (int)rr, 0 });
       }
       jobs.push_back(j);
     }

@@ -285,7 +285,8 @@ int autotune_batch(long n, long rows, int *best_variant, double *best_us)
     std::vector<BatchTile> q[kBatchQueues];
     make_batch_tiles(j, 0, q);
     std::vector<TileQueue> queues;
-    queues.push_back(TileQueue{ 0, 0, std::move(q[0]) });
+    if (!q[0].empty()) queues.push_back(TileQueue{ 0, 0, std::move(q[0]) });
+    if (!q[kMidQueue].empty()) queues.push_back(TileQueue{ kMidLine, 0, std::move(q[kMidQueue]) });
     GraphBatch gb;
     if ((st = gb.build(jobs, queues, s))) break;
     auto t0 = std::chrono::steady_clock::now();

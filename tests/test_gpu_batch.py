@@ -1,4 +1,4 @@
-"""Batched uneven jobs in one persistent launch: each job's result is the
+"""Batched uneven jobs in persistent launches: each job's result is the
 same alone and in the mix (bitwise: the batch runs the same device bodies as
 the standalone kernels), and matches the FP64 oracle to rel-L2 <= 1e-5."""
 import numpy as np
@@ -46,8 +46,10 @@ def test_batch_matches_oracle_and_standalone(ovx):
         b.add(xd, reps[n][0], yd)
     b.commit()
     print("tiles", b.tiles, "launches", b.launches, flush=True)
-    # persistent launch + one launch per four-step pass (line length 256)
-    assert b.launches == 4
+    # two persistent launches (1024-thread workgroups for the rows of 12288
+    # and 16384 points, 512-thread ones for the shorter rows) + one launch
+    # per four-step pass (line length 256)
+    assert b.launches == 5
     b()
     outs = [host(yd) for *_, yd in jobs]
     variants = b.job_variants()
@@ -75,16 +77,25 @@ def test_batch_matches_oracle_and_standalone(ovx):
         assert np.array_equal(host(yd), y), (n, rows)
 
 
-def test_batch_lds_only_single_launch(ovx):
-    jobs, reps = make_jobs(ovx, [(1024, 5), (16384, 2), (512, 70)], seed=40)
+@pytest.mark.parametrize("spec,launches", [
+    ([(1024, 5), (16384, 2), (512, 70)], 2),  # both persistent launches
+    ([(1024, 5), (8192, 3), (512, 70)], 1),   # 512-thread workgroups only
+    ([(12288, 3), (16384, 2)], 1),            # 1024-thread workgroups only
+])
+def test_batch_lds_only_persistent_launches(ovx, spec, launches):
+    jobs, reps = make_jobs(ovx, spec, seed=40)
     b = ovx.Batch()
     for n, rows, x, xd, yd in jobs:
         b.add(xd, reps[n][0], yd)
     b.commit()
-    assert b.launches == 1
+    assert b.launches == launches
     b()
-    for n, rows, x, xd, yd in jobs:
-        assert o.rel_l2(host(yd), o.fastconv64(x, reps[n][1])) < 1e-5
+    outs = [host(yd) for *_, yd in jobs]
+    for (n, rows, x, xd, yd), y in zip(jobs, outs):
+        assert o.rel_l2(y, o.fastconv64(x, reps[n][1])) < 1e-5, (n, rows)
+    b.plain()
+    for (n, rows, x, xd, yd), y in zip(jobs, outs):
+        assert np.array_equal(host(yd), y), (n, rows)
 
 
 def test_batch_four_step_only_and_mixed_line_lengths(ovx):
