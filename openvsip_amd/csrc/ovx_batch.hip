@@ -173,6 +173,11 @@ template <int NT, int PATH, int N>
 constexpr bool runs_in() { return batch_mid_routed(PATH, N) == (NT == kBatchMidThreads); }
 template <int NT, int L>
 constexpr bool ols_runs_in() { return batch_ols_mid_routed(L) == (NT == kBatchMidThreads); }
+static_assert(batch_mid_routed(BP_HIER, 8192) && !batch_mid_routed(BP_HIER, 16384) &&
+                  batch_mid_routed(BP_ROW, 6144) && !batch_mid_routed(BP_ROW, 12288) &&
+                  batch_mid_routed(BP_ROW, 256) && batch_ols_mid_routed(8192) && !batch_ols_mid_routed(16384),
+              "rows of 12288 and 16384 points (and segments of 16384) need the 1024-thread launch, all others "
+              "run in the 512-thread one");
 
 template <int NT, int N>
 __device__ __forceinline__ void run_hier(bool fwd_only, lds_job &j, int a, lds_ptr lds)
