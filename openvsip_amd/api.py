@@ -397,7 +397,8 @@ def init(device=0):
 
 
 class Batch:
-    """Uneven fastconv jobs packed into one persistent launch (graph replay)."""
+    """Uneven fastconv jobs packed into persistent tile-queue launches
+    (graph replay)."""
 
     def __init__(self):
         self._h = ctypes.c_void_p()

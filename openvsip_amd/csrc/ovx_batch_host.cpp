@@ -1,5 +1,5 @@
 // ovx_batch_host.cpp -- batches of uneven fastconv jobs (C-ABI part):
-// tile-queue construction, device tables, the persistent launch over the
+// tile-queue construction, device tables, the persistent launches over the
 // LDS-resident tiles and the four-step pass launches captured once into a
 // hipGraph and replayed, and the plain one-launch-per-job path it is
 // compared with.
@@ -28,7 +28,7 @@ static std::map<std::pair<long, int>, std::unique_ptr<LineFFT>> g_paths;
 // Variant the batch tiles of an (n, rows) job run: the autotuner's batch
 // entry for (n, rows) when it has one, else the default -- for the 3 * 2^k
 // lengths the 12-values-per-thread register-Stockham configuration (variant
-// 1), which fits the persistent kernel's 1024-thread register budget, for
+// 1), which fits the 1024-thread persistent kernel's register budget, for
 // the others the standalone default.
 int batch_variant(long n, long rows)
 {

@@ -3,8 +3,8 @@
 // Requests (fast convolution against a resident replica, forward Fftm,
 // FFT-based correlation against a prepared reference) arrive on a queue.  A
 // scheduler thread takes a free slot -- a stream with its own device tile
-// tables, four-step workspace and a hipGraph of tile-queue launches (one
-// persistent launch, one per four-step pass and line length) captured once
+// tables, four-step workspace and a hipGraph of tile-queue launches (the
+// persistent launches, one per four-step pass and line length) captured once
 // at start-up -- admits queued requests into it in
 // arrival order while their device working sets fit the HBM budget, uploads
 // the slot's tables and replays the graph.  A completion thread retires slots

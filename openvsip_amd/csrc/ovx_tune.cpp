@@ -12,7 +12,7 @@
 // (n, rows rounded up to a power of two).
 // Plans created with the default variant flag consult the cache.  Batches
 // and the server have their own entries: which tile body (hierarchical, or
-// the register-Stockham configuration the persistent kernel holds) runs the
+// the register-Stockham configuration the persistent kernels hold) runs the
 // rows of an (n, rows) job inside a batch, timed as a one-job batch replayed
 // from its graph; a batch tunes the keys it lacks when it is committed.  The
 // cache is a text file, one "n rows_bucket variant microseconds" line per
