@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--serve-saturation-jobs", type=int, default=3000)
     ap.add_argument("--serve-slots", type=int, default=3, help="batches in flight in the batched server")
     ap.add_argument("--serve-max-jobs", type=int, default=512, help="requests per batch in the batched server")
-    ap.add_argument("--serve-rounds", type=int, default=5,
+    ap.add_argument("--serve-rounds", type=int, default=9,
                     help="saturation rounds (a fresh trace each, run by every path; medians reported)")
     return ap.parse_args()
 
