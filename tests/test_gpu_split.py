@@ -95,8 +95,11 @@ def test_fft_split_by_reference(ovx, n):
         lib.ovx_plan_destroy(i)
 
 
-# rows, n, row gap: register Stockham, hierarchical and four-step paths
-CASES = [(2, 64, 0), (3, 512, 5), (7, 2048, 1), (16, 16384, 3), (4, 65536, 2), (5, 6144, 0)]
+# rows, n, row gap: register Stockham, hierarchical and four-step paths (the
+# four-step split passes run 32 lines per group: 256 x 256, 256 x 512 and
+# 512 x 512 matrices, the last in 1024-thread groups)
+CASES = [(2, 64, 0), (3, 512, 5), (7, 2048, 1), (16, 16384, 3), (4, 65536, 2), (5, 6144, 0), (3, 131072, 0),
+         (2, 262144, 1)]
 
 
 @pytest.mark.parametrize("rows,n,gap", CASES)
@@ -154,7 +157,7 @@ def test_fftm_split_columns_inverse(ovx, n, cols):
         lib.ovx_plan_destroy(h)
 
 
-@pytest.mark.parametrize("n", [2048, 16384, 65536])
+@pytest.mark.parametrize("n", [2048, 16384, 65536, 131072, 262144])
 @pytest.mark.parametrize("matrix", [False, True])
 def test_fastconv_split_equals_interleaved(ovx, n, matrix):
     import torch
