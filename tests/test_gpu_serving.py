@@ -27,7 +27,11 @@ def build_mix(ovx):
     rng = np.random.default_rng(11)
     specs = [("fc", 512, 40), ("fc", 2048, 17), ("fc", 3072, 9), ("fc", 16384, 5), ("fc", 65536, 3),
              ("fc", 12288, 6), ("fft", 1024, 12), ("fft", 6144, 7), ("fft", 65536, 2), ("fft", 512, 33),
-             ("corr", 65536, 1), ("fc", 8192, 30), ("fft", 16384, 4), ("corr", 65536, 1), ("fc", 4096, 64)]
+             ("corr", 65536, 1), ("fc", 8192, 30), ("fft", 16384, 4), ("corr", 65536, 1), ("fc", 4096, 64),
+             # every LDS-resident length of both kinds, in both persistent
+             # launches (512-thread workgroups up to 8192, 1024 above)
+             ("fc", 1024, 21), ("fc", 6144, 11), ("fft", 2048, 5), ("fft", 3072, 6), ("fft", 4096, 8),
+             ("fft", 8192, 3), ("fft", 12288, 2)]
     reps, items = {}, []
     ref = wl.cgauss(rng, 1024)
     cref = ovx.CorrReference("full", ref, 65536)
